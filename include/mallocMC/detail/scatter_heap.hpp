@@ -1,0 +1,751 @@
+/*
+ * ScatterAlloc heap core for gfx950: geometry, metadata layout and the
+ * per-lane allocation protocol of mallocMC's Scatter creation policy
+ * (src/include/mallocMC/creationPolicies/Scatter.hpp), executed with runtime
+ * geometry so one build serves every pagesize/regionsize/hash setting.
+ *
+ * Heap layout inside the pool (identical to Scatter.hpp:999-1014 so that heap
+ * offsets are bit-exact): numpages pages, then one 12-byte PTE per page
+ * {chunksize, count, bitmask}, then one uint32 fill level per region.  The
+ * three mutable heap scalars (_firstFreePageBased, _firstfreeblock,
+ * _pagebasedMutex, Scatter.hpp:269-271) live in a separate device block, one
+ * 128-byte L2 line each.
+ *
+ * The identity policy (`Ids`) supplies lane / wave / CU ids.  `WaveIds` reads
+ * the hardware; `SerialIds` pins them to the values of the reference's CPU
+ * backend (warpSize 1, all ids 0, mallocMC_utils.hpp:85-87), which defines
+ * the serialized request stream whose offsets must match the oracle.
+ *
+ * Divergences from the reference (undefined or unsafe there):
+ *   D1 heap < accessblocksize: number of access blocks clamped to 1 (the
+ *      reference divides by zero, Scatter.hpp:992-993).
+ *   D2 the second page-based scan treats index numpages as "not free" (the
+ *      reference reads one PTE past the end, Scatter.hpp:785-788, 835).
+ *   D3 free of a pointer whose page has chunksize 0 is ignored (division by
+ *      zero at Scatter.hpp:929).
+ *   Native fix (flag kFixPagesizeLeak, off by default): a chunk of exactly
+ *      pagesize is released through the chunked path, so its page does not
+ *      leak one slot (Scatter.hpp:906-908 vs :942-945).
+ */
+#pragma once
+
+#include "../mallocMC_utils.hpp"
+
+#include <cstdint>
+
+namespace mallocMC
+{
+    namespace detail
+    {
+        using u32 = std::uint32_t;
+        using u64 = std::uint64_t;
+
+        constexpr u64 kNullOffset = ~0ull;
+
+        enum ScatterFlags : u32
+        {
+            kFixPagesizeLeak = 1u << 0,
+        };
+
+        //! The three mutable heap scalars, each on its own 128-byte L2 line.
+        struct alignas(128) ScatterScalars
+        {
+            u32 firstFreePageBased;
+            u32 pad0[31];
+            u32 firstfreeblock;
+            u32 pad1[31];
+            u32 pagebasedMutex;
+            u32 pad2[31];
+        };
+
+        //! Runtime heap configuration (Scatter.hpp:56-85 + Shrink.hpp:60-66).
+        struct ScatterConfig
+        {
+            u32 pagesize = 4096u;
+            u32 regionsize = 16u;
+            u64 accessblocksize = 2ull * 1024u * 1024u * 1024u;
+            u32 wastefactor = 2u;
+            u32 resetfreedpages = 0u;
+            u32 hashingK = 38183u;
+            u32 hashingDistMP = 17497u;
+            u32 hashingDistWP = 1u;
+            u32 hashingDistWPRel = 1u;
+            u32 dataAlignment = 16u;
+            u32 flags = 0u;
+        };
+
+        //! Geometry computed by the init kernel (Scatter.hpp:962-1014).
+        struct ScatterGeometry
+        {
+            u32 numpages = 0;
+            u32 numregions = 0;
+            u32 accessblocks = 0;
+            u32 pagesperblock = 0;
+            u32 hierarchyThreshold = 0;
+            u32 minChunkSize = 0;
+            u32 maxOnPageMasks = 0;
+            u32 clampedBlocks = 0;
+            u64 ptesOffset = 0;
+            u64 regionsOffset = 0;
+            u64 usedBytes = 0;
+        };
+
+        MALLOCMC_HD u32 umin32(u32 a, u32 b)
+        {
+            return a < b ? a : b;
+        }
+        MALLOCMC_HD u32 umax32(u32 a, u32 b)
+        {
+            return a > b ? a : b;
+        }
+
+        //! Returns nullptr if the configuration is usable, else a message.
+        inline const char* validateConfig(const ScatterConfig& c)
+        {
+            if(c.pagesize < 128u || c.pagesize > (1u << 30))
+                return "pagesize must be in [128, 2^30]";
+            if(c.regionsize == 0u)
+                return "regionsize must be > 0";
+            if(c.accessblocksize == 0u)
+                return "accessblocksize must be > 0";
+            if(c.dataAlignment == 0u || (c.dataAlignment & (c.dataAlignment - 1u)) != 0u)
+                return "dataAlignment must be a power of two";
+            if(c.wastefactor == 0u)
+                return "wastefactor must be > 0";
+            return nullptr;
+        }
+
+        //! Geometry as initDeviceFunction computes it (Scatter.hpp:962-1014) plus
+        //! the derived constants of Scatter.hpp:173-187.
+        inline ScatterGeometry computeGeometry(const ScatterConfig& c, u64 memsize)
+        {
+            ScatterGeometry g;
+            const u32 P = c.pagesize;
+            g.hierarchyThreshold = (P - 2u * 4u) / 33u;
+            g.minChunkSize = (P - 32u * 4u) / (32u * 32u);
+            const u32 minSegment = 32u * g.minChunkSize + 4u;
+            const u32 onPage = g.minChunkSize > g.hierarchyThreshold ? 0u : (P + (minSegment - 1u)) / minSegment;
+            g.maxOnPageMasks = umin32(32u, onPage);
+
+            const u64 R = c.regionsize;
+            u32 numregions = static_cast<u32>(memsize / (R * (12u + static_cast<u64>(P)) + 4u));
+            u32 numpages = numregions * static_cast<u32>(R);
+            const u64 memorysize = static_cast<u64>(numpages) * P;
+            u64 numblocks = memorysize / c.accessblocksize;
+            if(numblocks == 0u)
+            {
+                numblocks = 1u;
+                g.clampedBlocks = 1u;
+            }
+            const u64 memoryperblock = memorysize / numblocks;
+            const u64 pagesperblock = memoryperblock / P;
+            const u64 regionsperblock = pagesperblock / R;
+            numregions = static_cast<u32>(numblocks * regionsperblock);
+            numpages = numregions * static_cast<u32>(R);
+            u64 ptes = static_cast<u64>(numpages) * P;
+            u64 regions = ptes + 12u * static_cast<u64>(numpages);
+            if(regions + 4u * static_cast<u64>(numregions) > memsize)
+            {
+                --numregions;
+                numpages = umin32(numregions * static_cast<u32>(R), numpages);
+                ptes = static_cast<u64>(numpages) * P;
+                regions = ptes + 12u * static_cast<u64>(numpages);
+            }
+            g.numpages = numpages;
+            g.numregions = numregions;
+            g.accessblocks = static_cast<u32>(numblocks);
+            g.pagesperblock = g.accessblocks ? numpages / g.accessblocks : 0u;
+            g.ptesOffset = ptes;
+            g.regionsOffset = regions;
+            g.usedBytes = regions + 4u * static_cast<u64>(numregions);
+            return g;
+        }
+
+        //! Lane / wave / CU identities of the executing lane.
+        struct WaveIds
+        {
+            static constexpr u32 warpSize = 64u;
+            static MALLOCMC_D u32 lane()
+            {
+                return hw::laneid();
+            }
+            static MALLOCMC_D u32 warp()
+            {
+                return hw::waveslot();
+            }
+            static MALLOCMC_D u32 sm()
+            {
+                return hw::cuid();
+            }
+            static MALLOCMC_D u64 active()
+            {
+                return hw::activemask();
+            }
+            static MALLOCMC_D u64 lanemaskLt()
+            {
+                return hw::lanemask_lt();
+            }
+        };
+
+        //! Identities of the reference's CPU backend: defines the serialized stream.
+        struct SerialIds
+        {
+            static constexpr u32 warpSize = 1u;
+            static MALLOCMC_D u32 lane()
+            {
+                return 0u;
+            }
+            static MALLOCMC_D u32 warp()
+            {
+                return 0u;
+            }
+            static MALLOCMC_D u32 sm()
+            {
+                return 0u;
+            }
+            static MALLOCMC_D u64 active()
+            {
+                return 1u;
+            }
+            static MALLOCMC_D u64 lanemaskLt()
+            {
+                return 0u;
+            }
+        };
+
+        /**
+         * By-value view of one heap: what a kernel needs to allocate and free.
+         * Trivially copyable; passed to kernels as an argument so the geometry
+         * sits in scalar registers.
+         */
+        struct ScatterHeap
+        {
+            char* pages = nullptr;
+            u32* ptes = nullptr;
+            u32* regions = nullptr;
+            ScatterScalars* scalars = nullptr;
+            u64 memsize = 0;
+            u32 pagesize = 0;
+            u32 pageShift = 0; // log2(pagesize) if a power of two, else 0
+            u32 regionsize = 0;
+            u32 wastefactor = 0;
+            u32 resetfreedpages = 0;
+            u32 hierarchyThreshold = 0;
+            u32 minChunkSize = 0;
+            u32 maxOnPageMasks = 0;
+            u32 paddedMinChunk = 0;
+            u32 alignMask = 0;
+            u32 hashingK = 0;
+            u32 hashingDistMP = 0;
+            u32 hashingDistWP = 0;
+            u32 hashingDistWPRel = 0;
+            u32 numpages = 0;
+            u32 numregions = 0;
+            u32 accessblocks = 0;
+            u32 pagesperblock = 0;
+            u32 flags = 0;
+
+            // ------------------------------------------------ addressing
+            MALLOCMC_D u32* pteChunksize(u32 p) const
+            {
+                return ptes + 3u * static_cast<u64>(p);
+            }
+            MALLOCMC_D u32* pteCount(u32 p) const
+            {
+                return ptes + 3u * static_cast<u64>(p) + 1u;
+            }
+            MALLOCMC_D u32* pteBitmask(u32 p) const
+            {
+                return ptes + 3u * static_cast<u64>(p) + 2u;
+            }
+            MALLOCMC_D char* pageData(u32 p) const
+            {
+                return pages + static_cast<u64>(p) * pagesize;
+            }
+            MALLOCMC_D u32 pageOf(u64 offset) const
+            {
+                return static_cast<u32>(pageShift ? (offset >> pageShift) : (offset / pagesize));
+            }
+            //! onPageMasksPosition (Scatter.hpp:315-318)
+            MALLOCMC_D u32* onPageMasks(u32 p, u32 nMasks) const
+            {
+                return reinterpret_cast<u32*>(pageData(p) + pagesize - 4u * nMasks);
+            }
+            //! Shrink::applyPadding (Shrink.hpp:132-137)
+            MALLOCMC_HD u32 pad(u32 bytes) const
+            {
+                return (bytes + alignMask) & ~alignMask;
+            }
+
+            // --------------------------------------------- bit utilities
+            //! nextspot (Scatter.hpp:293-304)
+            static MALLOCMC_D u32 nextspot(u32 bitfield, u32 spot, u32 spots)
+            {
+                const u32 low = (spot + 1u) == 32u ? 0u : (bitfield >> (spot + 1u));
+                const u32 high = bitfield << (spots - (spot + 1u));
+                const u32 sel = spots == 32u ? ~0u : ((1u << spots) - 1u);
+                const u32 rot = (high | low) & sel;
+                const u32 step = hw::ffs(static_cast<u32>(~rot));
+                return (spot + step) % spots;
+            }
+
+            //! usespot (Scatter.hpp:329-347): claim one free bit, -1 if full.
+            template<class Ids>
+            static MALLOCMC_D int usespot(u32* bitfield, u32 spots)
+            {
+                u32 spot = Ids::lane() % spots;
+                for(;;)
+                {
+                    const u32 mask = 1u << spot;
+                    const u32 old = atom::or_(bitfield, mask);
+                    if((old & mask) == 0u)
+                        return static_cast<int>(spot);
+                    if(hw::popc(old) >= spots)
+                        return -1;
+                    spot = nextspot(old, spot, spots);
+                }
+            }
+
+            //! calcAdditionalChunks (Scatter.hpp:365-383), signed reading.
+            MALLOCMC_D u32 calcAdditional(u32 fullsegments, u32 segmentsize, u32 chunksize) const
+            {
+                if(fullsegments == 32u)
+                    return 0u;
+                const long long rem = static_cast<long long>(pagesize)
+                    - static_cast<long long>(fullsegments) * static_cast<long long>(segmentsize) - 4ll;
+                const long long add = (rem < 0 ? 0ll : rem) / static_cast<long long>(chunksize);
+                return static_cast<u32>(add < 31 ? add : 31);
+            }
+
+            //! Number of chunks a page of the given chunk size holds (tryUsePage :494-510).
+            MALLOCMC_D u32 pageCapacity(u32 chunksize) const
+            {
+                if(chunksize <= hierarchyThreshold)
+                {
+                    const u32 segmentsize = chunksize * 32u + 4u;
+                    const u32 full = umin32(32u, pagesize / segmentsize);
+                    return full * 32u + calcAdditional(full, segmentsize, chunksize);
+                }
+                return umin32(pagesize / chunksize, 32u);
+            }
+
+            // ------------------------------------------- chunked path
+            //! addChunkHierarchy (Scatter.hpp:397-421)
+            template<class Ids>
+            MALLOCMC_D char* addChunkHierarchy(u32 chunksize, u32 fullsegments, u32 additional, u32 page) const
+            {
+                const u32 segments = fullsegments + (additional > 0u ? 1u : 0u);
+                u32 spot = Ids::lane() % segments;
+                const u32 mask = atom::load(pteBitmask(page));
+                if((mask & (1u << spot)) != 0u)
+                    spot = nextspot(mask, spot, segments);
+                const u32 tries = segments - hw::popc(mask);
+                u32* masks = onPageMasks(page, segments);
+                for(u32 i = 0; i < tries; ++i)
+                {
+                    const int hspot = usespot<Ids>(&masks[spot], spot < fullsegments ? 32u : additional);
+                    if(hspot != -1)
+                        return pageData(page) + static_cast<u64>(32u * spot + static_cast<u32>(hspot)) * chunksize;
+                    atom::or_(pteBitmask(page), 1u << spot);
+                    spot = nextspot(mask, spot, segments);
+                }
+                return nullptr;
+            }
+
+            //! addChunkNoHierarchy (Scatter.hpp:432-443)
+            template<class Ids>
+            MALLOCMC_D char* addChunkNoHierarchy(u32 chunksize, u32 page, u32 spots) const
+            {
+                const int spot = usespot<Ids>(pteBitmask(page), spots);
+                if(spot == -1)
+                    return nullptr;
+                return pageData(page) + static_cast<u64>(static_cast<u32>(spot)) * chunksize;
+            }
+
+            //! tryUsePage (Scatter.hpp:454-518); range check of allocChunked (:589-591).
+            template<class Ids>
+            MALLOCMC_D char* tryUsePage(u32 page, u32 chunksize, u32 bytes, u32 maxchunksize) const
+            {
+                char* chunk = nullptr;
+                const u32 filllevel = atom::add(pteCount(page), 1u);
+                bool tryAlloc = !resetfreedpages;
+                if(resetfreedpages && filllevel < pagesize)
+                {
+                    const u32 oldcs = atom::cas(pteChunksize(page), 0u, chunksize);
+                    if(oldcs == 0u || (oldcs >= bytes && oldcs <= maxchunksize))
+                        tryAlloc = true;
+                    if(oldcs != 0u)
+                        chunksize = oldcs;
+                }
+                if(tryAlloc)
+                {
+                    if(chunksize <= hierarchyThreshold)
+                    {
+                        const u32 segmentsize = chunksize * 32u + 4u;
+                        const u32 full = umin32(32u, pagesize / segmentsize);
+                        const u32 additional = calcAdditional(full, segmentsize, chunksize);
+                        if(filllevel < full * 32u + additional)
+                            chunk = addChunkHierarchy<Ids>(chunksize, full, additional, page);
+                    }
+                    else
+                    {
+                        const u32 chunksinpage = umin32(pagesize / chunksize, 32u);
+                        if(filllevel < chunksinpage)
+                            chunk = addChunkNoHierarchy<Ids>(chunksize, page, chunksinpage);
+                    }
+                }
+                if(chunk == nullptr)
+                    atom::sub(pteCount(page), 1u);
+                return chunk;
+            }
+
+            //! allocChunked (Scatter.hpp:527-637)
+            template<class Ids>
+            MALLOCMC_D char* allocChunked(u32 bytes) const
+            {
+                const u32 P = pagesize;
+                const u32 R = regionsize;
+                const u32 minAllocation = umax32(bytes, paddedMinChunk);
+                const u32 reloff = Ids::warpSize * minAllocation / P;
+                const u32 startInBlock
+                    = (minAllocation * hashingK + hashingDistMP * Ids::sm()
+                       + (hashingDistWP + hashingDistWPRel * reloff) * Ids::warp())
+                    % pagesperblock;
+                const u32 maxchunksize = umin32(P, umax32(wastefactor * bytes, paddedMinChunk));
+                const u32 globalStart = startInBlock + atom::load(&scalars->firstfreeblock) * pagesperblock;
+
+                u32 checklevel = R * 3u / 4u;
+                for(u32 finder = 0; finder < 2u; ++finder)
+                {
+                    u32 gp = globalStart;
+                    do
+                    {
+                        const u32 region = gp / R;
+                        const u32 fill = atom::load(&regions[region]);
+                        const u32 regionOffset = region * R;
+                        if(fill < checklevel)
+                        {
+                            u32 pir = gp;
+                            do
+                            {
+                                const u32 before = atom::cas(pteChunksize(pir), 0u, minAllocation);
+                                u32 use = 0u;
+                                if(before == 0u)
+                                    use = minAllocation;
+                                else if(before >= bytes && before <= maxchunksize)
+                                    use = before;
+                                if(use != 0u)
+                                {
+                                    char* res = tryUsePage<Ids>(pir, use, bytes, maxchunksize);
+                                    if(res != nullptr)
+                                        return res;
+                                }
+                                pir = regionOffset + ((pir + 1u) % R);
+                            } while(pir != gp);
+                            if(fill + 1u <= R)
+                                atom::cas(&regions[region], fill, fill + 1u);
+                        }
+                        gp = (gp + R) % numpages;
+                        if(gp % pagesperblock == 0u)
+                        {
+                            const u32 abid = gp / pagesperblock;
+                            if(abid > atom::load(&scalars->firstfreeblock))
+                                atom::store(&scalars->firstfreeblock, abid);
+                        }
+                    } while(gp != globalStart);
+                    checklevel = R + 1u;
+                }
+                return nullptr;
+            }
+
+            //! Page::init (Scatter.hpp:249-258): clear the on-page mask area.
+            MALLOCMC_D void pageInit(u32 page) const
+            {
+                u32* w = reinterpret_cast<u32*>(pageData(page) + pagesize - 4u * maxOnPageMasks);
+                for(u32 i = 0; i < maxOnPageMasks; ++i)
+                    atom::store(&w[i], 0u);
+            }
+
+            //! deallocChunked (Scatter.hpp:646-736)
+            template<class Ids>
+            MALLOCMC_D void deallocChunked(char* mem, u32 page, u32 chunksize) const
+            {
+                const u32 P = pagesize;
+                const u32 inpage = static_cast<u32>(mem - pageData(page));
+                if(chunksize <= hierarchyThreshold)
+                {
+                    const u32 segmentsize = chunksize * 32u + 4u;
+                    const u32 full = umin32(32u, P / segmentsize);
+                    const u32 additional = calcAdditional(full, segmentsize, chunksize);
+                    const u32 segment = inpage / (chunksize * 32u);
+                    const u32 within = (inpage - segment * (chunksize * 32u)) / chunksize;
+                    const u32 nMasks = full + (additional > 0u ? 1u : 0u);
+                    u32* masks = onPageMasks(page, nMasks);
+                    atom::and_(&masks[segment], ~(1u << within));
+                    atom::and_(pteBitmask(page), ~(1u << segment));
+                }
+                else
+                {
+                    const u32 segment = inpage / chunksize;
+                    atom::and_(pteBitmask(page), ~(1u << segment));
+                }
+
+                u32 oldfill;
+                if(resetfreedpages)
+                {
+                    oldfill = atom::cas(pteCount(page), 1u, P);
+                    if(oldfill == 1u)
+                    {
+                        pageInit(page);
+                        atom::cas(pteChunksize(page), chunksize, 0u);
+                        atom::fence_device();
+                        atom::sub(pteCount(page), P);
+                    }
+                    else
+                        oldfill = atom::sub(pteCount(page), 1u);
+                }
+                else
+                    oldfill = atom::sub(pteCount(page), 1u);
+
+                if(oldfill == P / 2u / chunksize)
+                {
+                    const u32 region = page / regionsize;
+                    atom::exch(&regions[region], 0u);
+                    const u32 block = page / pagesperblock;
+                    if(Ids::warp() + Ids::lane() == 0u)
+                        atom::min(&scalars->firstfreeblock, block);
+                }
+            }
+
+            // ---------------------------------------- page-based path
+            //! markpages (Scatter.hpp:745-764)
+            MALLOCMC_D bool markpages(u32 startpage, u32 npages, u32 bytes) const
+            {
+                u32 abortAt = 0xffffffffu;
+                for(u32 p = startpage; p < startpage + npages; ++p)
+                {
+                    if(atom::cas(pteChunksize(p), 0u, bytes) != 0u)
+                    {
+                        abortAt = p;
+                        break;
+                    }
+                }
+                if(abortAt == 0xffffffffu)
+                    return true;
+                for(u32 p = startpage; p < abortAt; ++p)
+                    atom::cas(pteChunksize(p), bytes, 0u);
+                return false;
+            }
+
+            //! allocPageBasedSingleRegion (Scatter.hpp:775-813), divergence D2.
+            MALLOCMC_D char* allocPageBasedSingleRegion(u32 startpage, u32 endpage, u32 bytes) const
+            {
+                const u32 pagestoalloc = divup(bytes, pagesize);
+                u32 freecount = 0;
+                bool leftFree = false;
+                for(u32 sp = startpage + 1u; sp > endpage;)
+                {
+                    --sp;
+                    const bool isFree = sp < numpages && atom::load(pteChunksize(sp)) == 0u;
+                    if(isFree)
+                    {
+                        if(++freecount == pagestoalloc)
+                        {
+                            if(markpages(sp, pagestoalloc, bytes))
+                            {
+                                if(!leftFree)
+                                    atom::cas(&scalars->firstFreePageBased, startpage, sp - 1u);
+                                return pageData(sp);
+                            }
+                        }
+                    }
+                    else
+                    {
+                        leftFree = true;
+                        freecount = 0;
+                    }
+                }
+                return nullptr;
+            }
+
+            //! allocPageBasedSingle (Scatter.hpp:824-840): global spin mutex.
+            MALLOCMC_D char* allocPageBasedSingle(u32 bytes) const
+            {
+                while(atom::exch(&scalars->pagebasedMutex, 1u) != 0u)
+                    __builtin_amdgcn_s_sleep(2);
+                const u32 spage = atom::load(&scalars->firstFreePageBased);
+                char* res = allocPageBasedSingleRegion(spage, 0u, bytes);
+                if(res == nullptr)
+                    res = allocPageBasedSingleRegion(numpages, spage, bytes);
+                atom::fence_device();
+                atom::exch(&scalars->pagebasedMutex, 0u);
+                return res;
+            }
+
+            //! allocPageBased (Scatter.hpp:848-866): one active lane at a time.
+            template<class Ids>
+            MALLOCMC_D char* allocPageBased(u32 bytes) const
+            {
+                char* res = nullptr;
+                const u64 mask = Ids::active();
+                const u32 num = hw::popc(mask);
+                const u32 localId = hw::popc(Ids::lanemaskLt() & mask);
+                for(u32 active = 0; active < num; ++active)
+                    if(active == localId)
+                        res = allocPageBasedSingle(bytes);
+                return res;
+            }
+
+            //! deallocPageBased (Scatter.hpp:874-886)
+            MALLOCMC_D void deallocPageBased(u32 page, u32 bytes) const
+            {
+                const u32 npages = divup(bytes, pagesize);
+                for(u32 p = page; p < page + npages; ++p)
+                    pageInit(p);
+                atom::fence_device();
+                for(u32 p = page; p < page + npages; ++p)
+                    atom::cas(pteChunksize(p), bytes, 0u);
+                atom::max(&scalars->firstFreePageBased, page + npages - 1u);
+            }
+
+            // ------------------------------------------------- entry
+            //! create (Scatter.hpp:895-912); bytes already padded.
+            template<class Ids>
+            MALLOCMC_D char* create(u32 bytes) const
+            {
+                if(bytes == 0u)
+                    return nullptr;
+                if(bytes <= pagesize)
+                    return allocChunked<Ids>(bytes);
+                return allocPageBased<Ids>(bytes);
+            }
+
+            //! destroy (Scatter.hpp:918-946) with divergence D3.
+            template<class Ids>
+            MALLOCMC_D void destroy(void* memv) const
+            {
+                if(memv == nullptr)
+                    return;
+                char* mem = static_cast<char*>(memv);
+                const u32 page = pageOf(static_cast<u64>(mem - pages));
+                const u32 chunksize = atom::load(pteChunksize(page));
+                if(chunksize == 0u)
+                    return;
+                const u32 inpage = static_cast<u32>(mem - pageData(page));
+                const u32 block = inpage / chunksize;
+                const u32 inblock = inpage - block * chunksize;
+                if(inblock != 0u)
+                {
+                    u32* counter = reinterpret_cast<u32*>(pageData(page) + static_cast<u64>(block) * chunksize);
+                    const u32 old = atom::sub(counter, 1u);
+                    if(old != 1u)
+                        return;
+                    mem = reinterpret_cast<char*>(counter);
+                }
+                if(chunksize < pagesize || (chunksize == pagesize && (flags & kFixPagesizeLeak)))
+                    deallocChunked<Ids>(mem, page, chunksize);
+                else
+                    deallocPageBased(page, chunksize);
+            }
+
+            // ------------------------------------------ slot counting
+            //! countFreeChunksInPage (Scatter.hpp:1112-1139)
+            MALLOCMC_D u32 countFreeChunksInPage(u32 page, u32 chunksize) const
+            {
+                const u32 filled = atom::load(pteCount(page));
+                return pageCapacity(chunksize) - filled;
+            }
+
+            //! getAvailaibleSlotsDeviceFunction (Scatter.hpp:1158-1219)
+            MALLOCMC_D u32 availableSlotsPartial(u64 slotSize, u32 gid, u32 stride) const
+            {
+                u32 slots = 0u;
+                if(slotSize < pagesize)
+                {
+                    const u32 maxchunksize = umin32(pagesize, wastefactor * static_cast<u32>(slotSize));
+                    for(u32 p = gid; p < numpages; p += stride)
+                    {
+                        u32 cs = atom::load(pteChunksize(p));
+                        if(cs >= slotSize && cs <= maxchunksize)
+                            slots += countFreeChunksInPage(p, cs);
+                        else if(cs == 0u)
+                        {
+                            cs = umax32(static_cast<u32>(slotSize), paddedMinChunk);
+                            slots += countFreeChunksInPage(p, cs);
+                        }
+                    }
+                }
+                else
+                {
+                    if(gid > 0u)
+                        return 0u;
+                    const u32 pagestoalloc = divup(static_cast<u32>(slotSize), pagesize);
+                    u32 freecount = 0u;
+                    for(u32 p = numpages; p > 0u;)
+                    {
+                        --p;
+                        if(atom::load(pteChunksize(p)) == 0u)
+                        {
+                            if(++freecount == pagestoalloc)
+                            {
+                                freecount = 0u;
+                                ++slots;
+                            }
+                        }
+                        else
+                            freecount = 0u;
+                    }
+                }
+                return slots;
+            }
+
+            MALLOCMC_D u64 offsetOf(const void* p) const
+            {
+                return p == nullptr ? kNullOffset : static_cast<u64>(static_cast<const char*>(p) - pages);
+            }
+        };
+
+        //! Builds the by-value view for a pool already laid out by computeGeometry.
+        inline ScatterHeap makeHeapView(
+            const ScatterConfig& c,
+            const ScatterGeometry& g,
+            void* pool,
+            u64 memsize,
+            ScatterScalars* scalars)
+        {
+            ScatterHeap h;
+            h.pages = static_cast<char*>(pool);
+            h.ptes = reinterpret_cast<u32*>(h.pages + g.ptesOffset);
+            h.regions = reinterpret_cast<u32*>(h.pages + g.regionsOffset);
+            h.scalars = scalars;
+            h.memsize = memsize;
+            h.pagesize = c.pagesize;
+            h.pageShift = 0u;
+            if((c.pagesize & (c.pagesize - 1u)) == 0u)
+            {
+                u32 s = 0;
+                while((1u << s) < c.pagesize)
+                    ++s;
+                h.pageShift = s;
+            }
+            h.regionsize = c.regionsize;
+            h.wastefactor = c.wastefactor;
+            h.resetfreedpages = c.resetfreedpages ? 1u : 0u;
+            h.hierarchyThreshold = g.hierarchyThreshold;
+            h.minChunkSize = g.minChunkSize;
+            h.maxOnPageMasks = g.maxOnPageMasks;
+            h.alignMask = c.dataAlignment - 1u;
+            h.paddedMinChunk = (g.minChunkSize + h.alignMask) & ~h.alignMask;
+            h.hashingK = c.hashingK;
+            h.hashingDistMP = c.hashingDistMP;
+            h.hashingDistWP = c.hashingDistWP;
+            h.hashingDistWPRel = c.hashingDistWPRel;
+            h.numpages = g.numpages;
+            h.numregions = g.numregions;
+            h.accessblocks = g.accessblocks;
+            h.pagesperblock = g.pagesperblock;
+            h.flags = c.flags;
+            return h;
+        }
+    } // namespace detail
+} // namespace mallocMC

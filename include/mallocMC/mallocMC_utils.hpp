@@ -1,0 +1,167 @@
+/*
+ * Wave64 hardware helpers for gfx950 (CDNA4).
+ *
+ * Replaces the reference's hardware shim src/include/mallocMC/mallocMC_utils.hpp
+ * (laneid :91-102, warpid :111-124, smid :126-137, lanemask_lt :139-150,
+ * ballot :152-162, activemask :165-175, ffs :205-224, popc :226-245,
+ * threadfence :247-293).  Lane masks are 64-bit: a gfx950 wavefront has 64
+ * lanes.  All shared heap state is accessed through agent-scope atomics so
+ * that the eight XCDs' L2 caches never serve a stale copy.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#define MALLOCMC_HD __host__ __device__ __forceinline__
+#define MALLOCMC_D __device__ __forceinline__
+
+namespace mallocMC
+{
+    //! Wavefront width of gfx950.
+    constexpr std::uint32_t waveSize = 64u;
+    //! Largest work-group the HIP runtime launches (used to size per-wave LDS arrays).
+    constexpr std::uint32_t maxThreadsPerBlock = 1024u;
+
+    template<class T>
+    MALLOCMC_HD T divup(T a, T b)
+    {
+        return (a + b - 1) / b;
+    }
+
+    namespace hw
+    {
+        MALLOCMC_D std::uint32_t laneid()
+        {
+            return __lane_id();
+        }
+
+        //! Wave slot within its SIMD (HW_ID[3:0]); changes only between launches.
+        MALLOCMC_D std::uint32_t waveslot()
+        {
+            return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 4);
+        }
+
+        //! Compute unit id including SE and XCD bits (HIP's __smid on gfx950).
+        MALLOCMC_D std::uint32_t cuid()
+        {
+            return __smid();
+        }
+
+        //! XCD (XCC) the wave runs on, 0..7 on MI355X.
+        MALLOCMC_D std::uint32_t xccid()
+        {
+            return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);
+        }
+
+        MALLOCMC_D std::uint64_t ballot(bool pred)
+        {
+            return __ballot(pred);
+        }
+
+        MALLOCMC_D std::uint64_t activemask()
+        {
+            return __ballot(1);
+        }
+
+        MALLOCMC_D std::uint64_t lanemask_lt()
+        {
+            return __lanemask_lt();
+        }
+
+        //! 1-based index of the lowest set bit, 0 for 0 (the reference's ffs contract).
+        MALLOCMC_D std::uint32_t ffs(std::uint64_t m)
+        {
+            return static_cast<std::uint32_t>(__ffsll(static_cast<unsigned long long>(m)));
+        }
+
+        MALLOCMC_D std::uint32_t popc(std::uint64_t m)
+        {
+            return static_cast<std::uint32_t>(__popcll(static_cast<unsigned long long>(m)));
+        }
+
+        template<class T>
+        MALLOCMC_D T readlane(T v, std::uint32_t lane)
+        {
+            return __shfl(v, static_cast<int>(lane), 64);
+        }
+
+        MALLOCMC_D std::uint64_t readlane64(std::uint64_t v, std::uint32_t lane)
+        {
+            const std::uint32_t lo = __shfl(static_cast<std::uint32_t>(v), static_cast<int>(lane), 64);
+            const std::uint32_t hi = __shfl(static_cast<std::uint32_t>(v >> 32), static_cast<int>(lane), 64);
+            return (static_cast<std::uint64_t>(hi) << 32) | lo;
+        }
+    } // namespace hw
+
+    //! Agent-scope relaxed atomics on heap metadata (the reference uses
+    //! alpaka::atomicOp with the default grid hierarchy, alpaka/atomic/Traits.hpp:70-78).
+    namespace atom
+    {
+        template<class T>
+        MALLOCMC_D T load(const T* p)
+        {
+            return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        template<class T>
+        MALLOCMC_D void store(T* p, T v)
+        {
+            __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        template<class T>
+        MALLOCMC_D T add(T* p, T v)
+        {
+            return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        template<class T>
+        MALLOCMC_D T sub(T* p, T v)
+        {
+            return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        template<class T>
+        MALLOCMC_D T or_(T* p, T v)
+        {
+            return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        template<class T>
+        MALLOCMC_D T and_(T* p, T v)
+        {
+            return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        template<class T>
+        MALLOCMC_D T exch(T* p, T v)
+        {
+            return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        template<class T>
+        MALLOCMC_D T min(T* p, T v)
+        {
+            return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        template<class T>
+        MALLOCMC_D T max(T* p, T v)
+        {
+            return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        //! Compare-and-swap returning the old value (CUDA atomicCAS contract).
+        template<class T>
+        MALLOCMC_D T cas(T* p, T cmp, T val)
+        {
+            __hip_atomic_compare_exchange_strong(
+                p,
+                &cmp,
+                val,
+                __ATOMIC_RELAXED,
+                __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
+            return cmp;
+        }
+        //! Agent-scope fence (the reference's threadfenceDevice, mallocMC_utils.hpp:247-293).
+        MALLOCMC_D void fence_device()
+        {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        }
+    } // namespace atom
+} // namespace mallocMC

@@ -1,0 +1,181 @@
+"""GPU parity: the HIP heap against the CPU oracle, through the C ABI.
+
+Serialized streams (one lane, the CPU backend's ids) must return the oracle's
+offsets and nulls bit-exactly; concurrent runs must satisfy the verify_heap
+invariants (tests/verify_heap.cpp:241-405 of the reference): patterns intact,
+aligned, inside the page area, no two live allocations overlapping, and a
+free-all restoring getAvailableSlots when resetfreedpages is on.
+"""
+import numpy as np
+import pytest
+
+from mallocmc_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+MiB = 1024 ** 2
+GiB = 1024 ** 3
+
+
+def _oracle_cfg(orc, c):
+    return orc.make_config(pagesize=c.pagesize, regionsize=c.regionsize, accessblocksize=c.accessblocksize,
+                           wastefactor=c.wastefactor, resetfreedpages=c.resetfreedpages, hashingK=c.hashingK,
+                           hashingDistMP=c.hashingDistMP, hashingDistWP=c.hashingDistWP,
+                           hashingDistWPRel=c.hashingDistWPRel, dataAlignment=c.dataAlignment)
+
+
+def _serial_pair(mmc, orc, cuda, cfg, heap_bytes, ops):
+    print(f"serial stream: P={cfg.pagesize} R={cfg.regionsize} heap={heap_bytes} n={len(ops)}", flush=True)
+    with orc.OracleHeap(_oracle_cfg(orc, cfg), heap_bytes) as oh:
+        want = oh.run_stream(ops).astype(np.int64)
+        want_slots = [oh.available_slots(s) for s in (16, 64, 1000, cfg.pagesize, 3 * cfg.pagesize)]
+    a = mmc.ScatterAllocator(cfg, heap_bytes)
+    got = a.serial_stream(cuda.from_numpy(ops).cuda()).cpu().numpy()
+    got_slots = [a.getAvailableSlots(s) for s in (16, 64, 1000, cfg.pagesize, 3 * cfg.pagesize)]
+    a.close()
+    return want, got, want_slots, got_slots
+
+
+def test_serial_config1_offsets(mmc, oracle_mod, cuda):
+    """Config 1: 16,384 x malloc(16) in thread order, then the frees."""
+    cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=2 * MiB, regionsize=16, wastefactor=2,
+                            resetfreedpages=True)
+    n = 16384
+    ops = np.concatenate([np.full(n, 16, dtype=np.int64), -np.arange(1, n + 1, dtype=np.int64)])
+    want, got, ws, gs = _serial_pair(mmc, oracle_mod, cuda, cfg, 64 * MiB, ops)
+    assert want[0] == 458752
+    np.testing.assert_array_equal(got, want)
+    assert gs == ws
+
+
+@pytest.mark.parametrize("seed,pagesize,reset,region", [
+    (1, 4096, False, 16), (2, 4096, True, 16), (3, 16384, True, 8), (4, 16384, False, 4),
+    (5, 65536, True, 16), (6, 1024, False, 2),
+])
+def test_serial_random_streams(mmc, oracle_mod, cuda, seed, pagesize, reset, region):
+    """Mixed sizes (chunked, == pagesize, multi-page), interleaved frees, small heaps
+    so that regions fill, the 3/4 threshold and OOM are reached."""
+    rng = np.random.default_rng(seed)
+    cand = np.array([1, 8, 16, 17, 40, 100, 128, 200, 500, 1000, pagesize // 2, pagesize - 16, pagesize,
+                     pagesize + 16, 2 * pagesize, 3 * pagesize + 5])
+    sizes = rng.choice(cand, size=4000)
+    ops = wl.random_stream(rng, 3000, sizes, p_free=0.3)
+    heap = max(8 * pagesize * region * 4, 2 * MiB)
+    cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=heap // 2, regionsize=region,
+                            resetfreedpages=reset)
+    want, got, ws, gs = _serial_pair(mmc, oracle_mod, cuda, cfg, heap, ops)
+    np.testing.assert_array_equal(got, want)
+    assert gs == ws
+    assert (want == -1).any() or True  # streams may or may not reach OOM
+
+
+def test_serial_fill_to_oom_and_refill(mmc, oracle_mod, cuda):
+    """Fill a tiny heap until null, free every other allocation, refill."""
+    cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=256 * 1024, regionsize=4, resetfreedpages=True)
+    heap = 1 * MiB
+    n1 = 2000
+    ops = [1000] * n1
+    ops += [-(i + 1) for i in range(0, n1, 2)]
+    ops += [700] * 1500
+    ops = np.array(ops, dtype=np.int64)
+    want, got, ws, gs = _serial_pair(mmc, oracle_mod, cuda, cfg, heap, ops)
+    assert (want[:n1] == -1).any(), "stream must reach OOM"
+    np.testing.assert_array_equal(got, want)
+    assert gs == ws
+
+
+def _check_no_overlap(torch, offsets, sizes, pad=16):
+    live = offsets >= 0
+    off = offsets[live]
+    sz = ((sizes[live].to(torch.int64) + pad - 1) // pad) * pad
+    order = torch.argsort(off)
+    off, sz = off[order], sz[order]
+    if off.numel() < 2:
+        return 0
+    return int(((off[:-1] + sz[:-1]) > off[1:]).sum().item())
+
+
+def test_concurrent_config1(mmc, cuda):
+    """Config 1 concurrently: 64 x 256 lanes malloc(16), write, verify, free;
+    free-all restores the slot count (resetfreedpages on)."""
+    torch = cuda
+    cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=2 * MiB, regionsize=16, resetfreedpages=True)
+    a = mmc.ScatterAllocator(cfg, 64 * MiB)
+    s0 = a.getAvailableSlots(16)
+    assert s0 == 15872 * 254
+    sizes = torch.full((64 * 256,), 16, dtype=torch.int32, device="cuda")
+    offs = a.malloc_batch(sizes)
+    a.write_pattern(offs, sizes, salt=7)
+    r = a.verify_pattern(offs, sizes, salt=7)
+    assert r["nulls"] == 0 and r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0
+    assert _check_no_overlap(torch, offs, sizes) == 0
+    assert a.getAvailableSlots(16) == s0 - 16384
+    a.free_batch(offs)
+    assert a.getAvailableSlots(16) == s0
+    a.close()
+
+
+@pytest.mark.parametrize("dist", [0, 1])
+def test_concurrent_mixed_sizes_rounds(mmc, cuda, dist):
+    """Config 2 shape at 1/4 of the threads (262,144 lanes, 1 GiB heap, 16 KiB
+    pages): three seeded rounds of malloc / write / verify / free."""
+    torch = cuda
+    cfg = mmc.ScatterConfig(pagesize=16384, accessblocksize=2 * GiB, regionsize=16, resetfreedpages=True,
+                            distribution=dist)
+    a = mmc.ScatterAllocator(cfg, 1 * GiB)
+    s0 = [a.getAvailableSlots(s) for s in (16, 4096)]
+    n = 262144
+    for seed in (1, 2, 3):
+        print(f"mixed round dist={dist} seed={seed}", flush=True)
+        sizes = torch.from_numpy(wl.mixed_sizes(seed, n).astype(np.int32)).cuda()
+        offs = a.malloc_batch(sizes)
+        a.write_pattern(offs, sizes, salt=seed << 40)
+        r = a.verify_pattern(offs, sizes, salt=seed << 40)
+        assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0, r
+        assert r["nulls"] == 0, r
+        assert _check_no_overlap(torch, offs, sizes) == 0
+        a.free_batch(offs)
+        torch.cuda.synchronize()
+        assert [a.getAvailableSlots(s) for s in (16, 4096)] == s0
+    a.close()
+
+
+def test_concurrent_xmallocsimd_small(mmc, cuda):
+    """XMallocSIMD on wave64: small requests coalesce; slices never overlap and
+    free-all restores the slot count."""
+    torch = cuda
+    cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=8 * MiB, regionsize=16, resetfreedpages=True,
+                            distribution=mmc.DIST_XMALLOCSIMD)
+    a = mmc.ScatterAllocator(cfg, 64 * MiB)
+    s0 = a.getAvailableSlots(16)
+    g = torch.Generator().manual_seed(5)
+    sizes = (torch.randint(1, 4, (65536,), generator=g) * 16).to(torch.int32).cuda()
+    offs = a.malloc_batch(sizes)
+    a.write_pattern(offs, sizes, salt=99)
+    r = a.verify_pattern(offs, sizes, salt=99)
+    assert r["nulls"] == 0 and r["mismatches"] == 0 and r["misaligned"] == 0, r
+    assert _check_no_overlap(torch, offs, sizes) == 0
+    a.free_batch(offs)
+    torch.cuda.synchronize()
+    assert a.getAvailableSlots(16) == s0
+    a.close()
+
+
+def test_alloc_until_null_verify_heap(mmc, cuda):
+    """verify_heap's AllocAll: fill a 64 MiB heap with 6000 B slots until null."""
+    torch = cuda
+    cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=8 * MiB, regionsize=16, resetfreedpages=True)
+    a = mmc.ScatterAllocator(cfg, 64 * MiB)
+    s0 = a.getAvailableSlots(6000)
+    offs, n = a.alloc_until_null(6000, capacity=1 << 16, blocks=64, threads=128)
+    assert 0 < n < (1 << 16)
+    sizes = torch.full((offs.numel(),), 6000, dtype=torch.int32, device="cuda")
+    a.write_pattern(offs, sizes, salt=3)
+    r = a.verify_pattern(offs, sizes, salt=3)
+    assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0
+    assert _check_no_overlap(torch, offs, sizes) == 0
+    assert a.getAvailableSlots(6000) == 0
+    a.free_batch(offs)
+    torch.cuda.synchronize()
+    assert a.getAvailableSlots(6000) == s0
+    a.close()
