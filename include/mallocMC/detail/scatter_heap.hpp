@@ -23,6 +23,11 @@
  *      reference reads one PTE past the end, Scatter.hpp:785-788, 835).
  *   D3 free of a pointer whose page has chunksize 0 is ignored (division by
  *      zero at Scatter.hpp:929).
+ *   D4 a page emptied by concurrent frees is always reset (see deallocChunked).
+ *   D5 a lane that formats a page during its concurrent reset waits for the
+ *      reset and then uses the page (see tryUsePage).
+ *   D4 and D5 make the same state transitions as the reference on serialized
+ *   streams; they close races the reference leaves open under concurrency.
  *   Native fix (flag kFixPagesizeLeak, off by default): a chunk of exactly
  *      pagesize is released through the chunked path, so its page does not
  *      leak one slot (Scatter.hpp:906-908 vs :942-945).
@@ -363,40 +368,56 @@ namespace mallocMC
             }
 
             //! tryUsePage (Scatter.hpp:454-518); range check of allocChunked (:589-591).
+            //! `formatted`: this lane set the page's chunk size from 0 just before
+            //! (divergence D5 below).
             template<class Ids>
-            MALLOCMC_D char* tryUsePage(u32 page, u32 chunksize, u32 bytes, u32 maxchunksize) const
+            MALLOCMC_D char* tryUsePage(u32 page, u32 chunksize, u32 bytes, u32 maxchunksize, bool formatted) const
             {
-                char* chunk = nullptr;
-                const u32 filllevel = atom::add(pteCount(page), 1u);
-                bool tryAlloc = !resetfreedpages;
-                if(resetfreedpages && filllevel < pagesize)
+                for(;;)
                 {
-                    const u32 oldcs = atom::cas(pteChunksize(page), 0u, chunksize);
-                    if(oldcs == 0u || (oldcs >= bytes && oldcs <= maxchunksize))
-                        tryAlloc = true;
-                    if(oldcs != 0u)
-                        chunksize = oldcs;
-                }
-                if(tryAlloc)
-                {
-                    if(chunksize <= hierarchyThreshold)
+                    char* chunk = nullptr;
+                    const u32 filllevel = atom::add(pteCount(page), 1u);
+                    bool tryAlloc = !resetfreedpages;
+                    if(resetfreedpages && filllevel < pagesize)
                     {
-                        const u32 segmentsize = chunksize * 32u + 4u;
-                        const u32 full = umin32(32u, pagesize / segmentsize);
-                        const u32 additional = calcAdditional(full, segmentsize, chunksize);
-                        if(filllevel < full * 32u + additional)
-                            chunk = addChunkHierarchy<Ids>(chunksize, full, additional, page);
+                        const u32 oldcs = atom::cas(pteChunksize(page), 0u, chunksize);
+                        if(oldcs == 0u || (oldcs >= bytes && oldcs <= maxchunksize))
+                            tryAlloc = true;
+                        if(oldcs != 0u)
+                            chunksize = oldcs;
                     }
-                    else
+                    if(tryAlloc)
                     {
-                        const u32 chunksinpage = umin32(pagesize / chunksize, 32u);
-                        if(filllevel < chunksinpage)
-                            chunk = addChunkNoHierarchy<Ids>(chunksize, page, chunksinpage);
+                        if(chunksize <= hierarchyThreshold)
+                        {
+                            const u32 segmentsize = chunksize * 32u + 4u;
+                            const u32 full = umin32(32u, pagesize / segmentsize);
+                            const u32 additional = calcAdditional(full, segmentsize, chunksize);
+                            if(filllevel < full * 32u + additional)
+                                chunk = addChunkHierarchy<Ids>(chunksize, full, additional, page);
+                        }
+                        else
+                        {
+                            const u32 chunksinpage = umin32(pagesize / chunksize, 32u);
+                            if(filllevel < chunksinpage)
+                                chunk = addChunkNoHierarchy<Ids>(chunksize, page, chunksinpage);
+                        }
                     }
+                    if(chunk == nullptr)
+                        atom::sub(pteCount(page), 1u);
+                    // D5: the lane formatted a page whose reset (count locked at
+                    // pagesize, Scatter.hpp:686-711) was still running.  Leaving
+                    // would strand the page formatted but empty; the reset is a
+                    // short straight-line section, so wait for it and use the page.
+                    // Never taken on a serialized stream (count stays < pagesize).
+                    if(chunk == nullptr && formatted && resetfreedpages && filllevel >= pagesize)
+                    {
+                        while(atom::load(pteCount(page)) >= pagesize)
+                            __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    return chunk;
                 }
-                if(chunk == nullptr)
-                    atom::sub(pteCount(page), 1u);
-                return chunk;
             }
 
             //! allocChunked (Scatter.hpp:527-637)
@@ -436,7 +457,7 @@ namespace mallocMC
                                     use = before;
                                 if(use != 0u)
                                 {
-                                    char* res = tryUsePage<Ids>(pir, use, bytes, maxchunksize);
+                                    char* res = tryUsePage<Ids>(pir, use, bytes, maxchunksize, before == 0u);
                                     if(res != nullptr)
                                         return res;
                                 }
@@ -490,22 +511,20 @@ namespace mallocMC
                     atom::and_(pteBitmask(page), ~(1u << segment));
                 }
 
-                u32 oldfill;
-                if(resetfreedpages)
+                // D4: the reference locks with CAS(count, 1 -> P) and decrements
+                // otherwise (Scatter.hpp:686-716); two concurrent frees can both
+                // miss the lock and leave the empty page formatted.  Here the lane
+                // whose decrement empties the page takes the lock with
+                // CAS(count, 0 -> P).  On a serialized stream both forms make the
+                // same state transitions and return the same old fill level.
+                const u32 oldfill = atom::sub(pteCount(page), 1u);
+                if(resetfreedpages && oldfill == 1u && atom::cas(pteCount(page), 0u, P) == 0u)
                 {
-                    oldfill = atom::cas(pteCount(page), 1u, P);
-                    if(oldfill == 1u)
-                    {
-                        pageInit(page);
-                        atom::cas(pteChunksize(page), chunksize, 0u);
-                        atom::fence_device();
-                        atom::sub(pteCount(page), P);
-                    }
-                    else
-                        oldfill = atom::sub(pteCount(page), 1u);
+                    pageInit(page);
+                    atom::cas(pteChunksize(page), chunksize, 0u);
+                    atom::fence_device();
+                    atom::sub(pteCount(page), P);
                 }
-                else
-                    oldfill = atom::sub(pteCount(page), 1u);
 
                 if(oldfill == P / 2u / chunksize)
                 {

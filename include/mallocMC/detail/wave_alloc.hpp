@@ -1,0 +1,540 @@
+/*
+ * Wave-cooperative allocation on the Scatter heap (gfx950, wave64).
+ *
+ * The reference serves every lane on its own (Scatter.hpp:527-637): each lane
+ * hashes to a page, CASes the chunk size of every page it visits, reserves
+ * one slot with atomicAdd(count) and claims one bit with atomicOr.  Here the
+ * 64 lanes of a wavefront cooperate on the same heap metadata:
+ *
+ *   * lanes are grouped into size classes whose members can share one chunk
+ *     size c while respecting the wastefactor rule b <= c <= max(WF*b,
+ *     minChunk) of Scatter.hpp:539-545, 589-591 (octaves of the request when
+ *     WF >= 2, exact sizes otherwise);
+ *   * a region is examined with one coalesced load of its PTEs (lane j reads
+ *     page j of the region) instead of one CAS per visited page;
+ *   * a page serves the whole group with one atomicAdd(count, k) reservation
+ *     and one vector atomicOr per touched 32-chunk mask word; chunk indices
+ *     are handed to lanes by ballot rank;
+ *   * frees are grouped by page: one atomicAnd per touched mask word, one
+ *     atomicAnd on the segment hints and one atomicSub(count, k).
+ *
+ * The metadata protocol is the reference's (same PTE fields, on-page masks,
+ * reset lock at count == pagesize, region fill levels, access-block cursor),
+ * so per-lane calls (scatter_heap.hpp) and wave calls can be mixed on one
+ * heap.  Requests above a page and coalesced XMallocSIMD slices use the
+ * per-lane path.
+ *
+ * Precondition: every function here is called by all 64 lanes of the wave
+ * in uniform control flow (lanes without work pass 0 / nullptr).
+ */
+#pragma once
+
+#include "scatter_heap.hpp"
+
+namespace mallocMC
+{
+    namespace detail
+    {
+        //! Tuning knobs of the wave path (autotuned by the batched runner).
+        struct WaveTuning
+        {
+            //! Maximum lanes one group claim serves (1 = per-lane like the reference).
+            u32 aggregation = 64u;
+            //! Spread start pages by XCD: each XCD hashes into its own slice of
+            //! every access block, so its atomics concentrate on its own lines.
+            u32 xcdSlices = 0u;
+        };
+
+        namespace wave
+        {
+            MALLOCMC_D u32 max(u32 v)
+            {
+                for(int d = 32; d > 0; d >>= 1)
+                {
+                    const u32 o = __shfl_xor(v, d, 64);
+                    v = v > o ? v : o;
+                }
+                return v;
+            }
+            MALLOCMC_D u32 min(u32 v)
+            {
+                for(int d = 32; d > 0; d >>= 1)
+                {
+                    const u32 o = __shfl_xor(v, d, 64);
+                    v = v < o ? v : o;
+                }
+                return v;
+            }
+            MALLOCMC_D u32 bor(u32 v)
+            {
+                for(int d = 32; d > 0; d >>= 1)
+                    v |= __shfl_xor(v, d, 64);
+                return v;
+            }
+            //! Value of lane `l` (l uniform).
+            MALLOCMC_D u32 lane(u32 v, u32 l)
+            {
+                return __builtin_amdgcn_readlane(v, l);
+            }
+            //! Position (0-based) of the r-th set bit of x; x must have > r bits.
+            MALLOCMC_D u32 selectBit(u32 x, u32 r)
+            {
+                u32 pos = 0u;
+                for(u32 step = 16u; step > 0u; step >>= 1)
+                {
+                    const u64 below = (static_cast<u64>(1u) << (pos + step)) - 1u;
+                    if(static_cast<u32>(hw::popc(x & below)) <= r)
+                        pos += step;
+                }
+                return pos;
+            }
+            //! The lowest `n` set bits of x.
+            MALLOCMC_D u32 lowestBits(u32 x, u32 n)
+            {
+                u32 out = 0u;
+                for(u32 i = 0; i < n && x != 0u; ++i)
+                {
+                    const u32 b = x & (0u - x);
+                    out |= b;
+                    x ^= b;
+                }
+                return out;
+            }
+        } // namespace wave
+
+        //! Size class used to group lanes; equal class => one chunk size c
+        //! (the class maximum) satisfies b <= c <= max(WF*b, minChunk) for
+        //! every member b.
+        MALLOCMC_D u32 sizeClass(const ScatterHeap& h, u32 b)
+        {
+            if(h.paddedMinChunk != 0u && b <= h.paddedMinChunk)
+                return 0u;
+            if(h.wastefactor < 2u)
+                return b;
+            const u32 base = h.paddedMinChunk != 0u ? h.paddedMinChunk : 16u;
+            const u32 q = (b - 1u) / base;
+            if(q == 0u)
+                return 0x80000000u | b; // below the octave base: exact size
+            // octaves above the base: (base*2^(j-1), base*2^j]
+            return 1u + (31u - static_cast<u32>(__builtin_clz(q)));
+        }
+
+        struct WaveAlloc
+        {
+            const ScatterHeap& h;
+            WaveTuning t;
+
+            //! Claims up to popc(mask) chunks of size cs on `page` for the group
+            //! lanes in `need` (a subset of the wave).  Returns the lanes served;
+            //! each served lane's `out` gets its chunk.  `formatted`: this wave
+            //! just set the page's chunk size from 0.
+            MALLOCMC_D u64 claimOnPage(u32 page, u32 cs, u32 bmax, u32 maxcs, u64 need, bool formatted,
+                                       char*& out) const
+            {
+                const u32 me = hw::laneid();
+                const u32 P = h.pagesize;
+                for(;;)
+                {
+                    const u32 k = hw::popc(need);
+                    u32 old = 0u;
+                    if(me == 0u)
+                        old = atom::add(h.pteCount(page), k);
+                    old = wave::lane(old, 0u);
+                    if(h.resetfreedpages)
+                    {
+                        if(old >= P)
+                        {
+                            // page locked by a concurrent reset
+                            if(me == 0u)
+                                atom::sub(h.pteCount(page), k);
+                            if(formatted)
+                            {
+                                // D5 of scatter_heap.hpp, for the wave
+                                if(me == 0u)
+                                    while(atom::load(h.pteCount(page)) >= P)
+                                        __builtin_amdgcn_s_sleep(1);
+                                __builtin_amdgcn_wave_barrier();
+                                continue;
+                            }
+                            return 0u;
+                        }
+                        u32 oldcs = 0u;
+                        if(me == 0u)
+                            oldcs = atom::cas(h.pteChunksize(page), 0u, cs);
+                        oldcs = wave::lane(oldcs, 0u);
+                        if(oldcs != 0u && oldcs != cs)
+                        {
+                            if(oldcs >= bmax && oldcs <= maxcs)
+                                cs = oldcs;
+                            else
+                            {
+                                if(me == 0u)
+                                    atom::sub(h.pteCount(page), k);
+                                return 0u;
+                            }
+                        }
+                    }
+                    const u32 cap = h.pageCapacity(cs);
+                    const u32 grant = old < cap ? umin32(k, cap - old) : 0u;
+                    u32 got = 0u;
+                    u64 served = 0u;
+                    if(grant != 0u)
+                        served = claimBits(page, cs, grant, need, got, out);
+                    if(got < k && me == 0u)
+                        atom::sub(h.pteCount(page), k - got);
+                    return served;
+                }
+            }
+
+            //! Claims `grant` free bits on the page's mask words and hands the
+            //! chunks to the lanes of `need` in rank order.  Returns lanes served.
+            MALLOCMC_D u64 claimBits(u32 page, u32 cs, u32 grant, u64 need, u32& gotTotal, char*& out) const
+            {
+                const u32 me = hw::laneid();
+                const u32 P = h.pagesize;
+                const bool hier = cs <= h.hierarchyThreshold;
+                u32 nWords, full = 0u, additional = 0u;
+                u32* words;
+                if(hier)
+                {
+                    const u32 segmentsize = cs * 32u + 4u;
+                    full = umin32(32u, P / segmentsize);
+                    additional = h.calcAdditional(full, segmentsize, cs);
+                    nWords = full + (additional > 0u ? 1u : 0u);
+                    words = h.onPageMasks(page, nWords);
+                }
+                else
+                {
+                    nWords = 1u;
+                    words = h.pteBitmask(page);
+                }
+                const u32 myRank = hw::popc(need & hw::lanemask_lt());
+                const bool inNeed = ((need >> me) & 1u) != 0u;
+                u64 served = 0u;
+                u32 remaining = grant;
+                gotTotal = 0u;
+                // hint mask: segments known full (hierarchical pages only)
+                u32 hints = 0u;
+                if(hier)
+                {
+                    if(me == 0u)
+                        hints = atom::load(h.pteBitmask(page));
+                    hints = wave::lane(hints, 0u);
+                }
+                for(u32 pass = 0; pass < 3u && remaining > 0u; ++pass)
+                {
+                    // lane w < nWords snapshots mask word w
+                    const bool owner = me < nWords;
+                    u32 spots = 0u, snap = 0u;
+                    if(owner)
+                    {
+                        spots = hier ? (me < full ? 32u : additional) : umin32(P / cs, 32u);
+                        const bool skipHint = hier && pass == 0u && ((hints >> me) & 1u);
+                        snap = skipHint ? ~0u : atom::load(&words[me]);
+                    }
+                    const u32 sel = spots == 32u ? ~0u : ((1u << spots) - 1u);
+                    const u32 freeBits = owner ? (~snap & sel) : 0u;
+                    const u32 freeCnt = hw::popc(freeBits);
+                    // exclusive prefix of free counts over words 0..me-1
+                    u32 prefix = 0u;
+                    {
+                        u32 v = freeCnt;
+                        for(int d = 1; d < 64; d <<= 1)
+                        {
+                            const u32 o = __shfl_up(v, static_cast<unsigned>(d), 64);
+                            if(me >= static_cast<u32>(d))
+                                v += o;
+                        }
+                        prefix = v - freeCnt;
+                    }
+                    u32 take = 0u;
+                    if(owner && prefix < remaining)
+                        take = umin32(freeCnt, remaining - prefix);
+                    const u32 choose = wave::lowestBits(freeBits, take);
+                    u32 gotBits = 0u;
+                    if(choose != 0u)
+                    {
+                        const u32 old = atom::or_(&words[me], choose);
+                        gotBits = choose & ~old;
+                        if(hier && ((old | choose) & sel) == sel)
+                            atom::or_(h.pteBitmask(page), 1u << me);
+                    }
+                    // hand out the chunks word by word, in rank order
+                    u64 words64 = hw::ballot(gotBits != 0u);
+                    while(words64 != 0u)
+                    {
+                        const u32 w = hw::ffs(words64) - 1u;
+                        words64 &= words64 - 1u;
+                        const u32 bits = wave::lane(gotBits, w);
+                        const u32 n = hw::popc(bits);
+                        if(inNeed && myRank >= gotTotal && myRank < gotTotal + n)
+                        {
+                            const u32 idx = 32u * w + wave::selectBit(bits, myRank - gotTotal);
+                            out = h.pageData(page) + static_cast<u64>(idx) * cs;
+                        }
+                        gotTotal += n;
+                    }
+                    remaining = grant - gotTotal;
+                }
+                // lanes of `need` with rank < gotTotal were served
+                served = hw::ballot(inNeed && myRank < gotTotal);
+                return served;
+            }
+
+            //! Group allocation of chunk size c for the lanes in `group`.
+            MALLOCMC_D u64 allocGroup(u64 group, u32 c, u32 bmax, u32 maxcs, char*& out) const
+            {
+                const u32 me = hw::laneid();
+                const u32 R = h.regionsize;
+                const u32 ppb = h.pagesperblock;
+                const u32 numpages = h.numpages;
+                u32 hashBase = c * h.hashingK + h.hashingDistMP * hw::cuid()
+                    + (h.hashingDistWP + h.hashingDistWPRel * (64u * c / h.pagesize)) * hw::waveslot();
+                u32 startInBlock;
+                if(t.xcdSlices > 1u && ppb >= t.xcdSlices * R)
+                {
+                    const u32 slice = ppb / t.xcdSlices;
+                    startInBlock = (hw::xccid() % t.xcdSlices) * slice + hashBase % slice;
+                }
+                else
+                    startInBlock = hashBase % ppb;
+                u32 ffb = 0u;
+                if(me == 0u)
+                    ffb = atom::load(&h.scalars->firstfreeblock);
+                ffb = wave::lane(ffb, 0u);
+                const u32 globalStart = startInBlock + ffb * ppb;
+
+                u64 need = group;
+                u32 checklevel = R * 3u / 4u;
+                for(u32 finder = 0; finder < 2u && need != 0u; ++finder)
+                {
+                    u32 gp = globalStart;
+                    do
+                    {
+                        const u32 region = gp / R;
+                        const u32 regionOffset = region * R;
+                        u32 fill = 0u;
+                        if(me == 0u)
+                            fill = atom::load(&h.regions[region]);
+                        fill = wave::lane(fill, 0u);
+                        if(fill < checklevel)
+                        {
+                            const u64 before = need;
+                            for(u32 base = 0; base < R && need != 0u; base += 64u)
+                            {
+                                const u32 j = base + me;
+                                u32 pcs = 0u, pcount = 0u;
+                                const bool valid = j < R;
+                                if(valid)
+                                {
+                                    pcs = atom::load(h.pteChunksize(regionOffset + j));
+                                    pcount = atom::load(h.pteCount(regionOffset + j));
+                                }
+                                bool cand = false;
+                                if(valid)
+                                {
+                                    if(pcs == 0u)
+                                        cand = true;
+                                    else if(pcs >= bmax && pcs <= maxcs && pcs <= h.pagesize)
+                                        cand = pcount < h.pageCapacity(pcs);
+                                }
+                                u64 cands = hw::ballot(cand);
+                                // cyclic order from the hashed page within the region
+                                const u32 rot = (gp - regionOffset) >= base ? (gp - regionOffset) - base : 0u;
+                                while(cands != 0u && need != 0u)
+                                {
+                                    const u64 hi = rot < 64u ? (cands >> rot) << rot : 0u;
+                                    const u64 pick = hi != 0u ? hi : cands;
+                                    const u32 l = hw::ffs(pick) - 1u;
+                                    cands &= ~(1ull << l);
+                                    const u32 page = regionOffset + base + l;
+                                    u32 cs = wave::lane(pcs, l);
+                                    bool formatted = false;
+                                    if(cs == 0u)
+                                    {
+                                        u32 old = 0u;
+                                        if(me == 0u)
+                                            old = atom::cas(h.pteChunksize(page), 0u, c);
+                                        old = wave::lane(old, 0u);
+                                        if(old == 0u)
+                                        {
+                                            cs = c;
+                                            formatted = true;
+                                        }
+                                        else if(old >= bmax && old <= maxcs)
+                                            cs = old;
+                                        else
+                                            continue;
+                                    }
+                                    const u64 served = claimOnPage(page, cs, bmax, maxcs, need, formatted, out);
+                                    need &= ~served;
+                                }
+                            }
+                            if(need == before && fill + 1u <= R && me == 0u)
+                                atom::cas(&h.regions[region], fill, fill + 1u);
+                        }
+                        gp = (gp + R) % numpages;
+                        if(gp % ppb == 0u && me == 0u)
+                        {
+                            const u32 abid = gp / ppb;
+                            if(abid > atom::load(&h.scalars->firstfreeblock))
+                                atom::store(&h.scalars->firstfreeblock, abid);
+                        }
+                    } while(gp != globalStart && need != 0u);
+                    checklevel = R + 1u;
+                }
+                return group & ~need;
+            }
+
+            //! Wave malloc: every lane passes its (unpadded) request size.
+            template<class Dist>
+            MALLOCMC_D void* malloc(u32 bytesIn) const
+            {
+                const u32 me = hw::laneid();
+                Dist dist;
+                const u32 b = dist.collect(h, h.pad(bytesIn));
+                char* result = nullptr;
+                bool want = b != 0u;
+                if(want && b > h.pagesize)
+                {
+                    result = h.allocPageBased<WaveIds>(b);
+                    want = false;
+                }
+                u64 pending = hw::ballot(want);
+                const u32 cls = want ? sizeClass(h, b) : 0xffffffffu;
+                const u32 minAlloc = umax32(b, h.paddedMinChunk);
+                while(pending != 0u)
+                {
+                    const u32 leader = hw::ffs(pending) - 1u;
+                    const u32 lcls = wave::lane(cls, leader);
+                    const bool inGroup = ((pending >> me) & 1u) != 0u && cls == lcls;
+                    u64 group = hw::ballot(inGroup);
+                    // cap the group at the aggregation width
+                    if(t.aggregation < 64u && hw::popc(group) > t.aggregation)
+                    {
+                        u64 g = 0u, tmp = group;
+                        for(u32 i = 0; i < t.aggregation; ++i)
+                        {
+                            const u64 lo = tmp & (0u - tmp);
+                            g |= lo;
+                            tmp ^= lo;
+                        }
+                        group = g;
+                    }
+                    const bool member = ((group >> me) & 1u) != 0u;
+                    const u32 c = wave::max(member ? minAlloc : 0u);
+                    const u32 bmax = wave::max(member ? b : 0u);
+                    const u32 bmin = wave::min(member ? b : 0xffffffffu);
+                    const u32 maxcs = umin32(h.pagesize, umax32(h.wastefactor * bmin, h.paddedMinChunk));
+                    char* mine = nullptr;
+                    const u64 served = allocGroup(group, c, bmax, maxcs, mine);
+                    if(member && ((served >> me) & 1u))
+                        result = mine;
+                    // lanes of the group not served: the heap has no room for them
+                    pending &= ~group;
+                }
+                void* r = result;
+                if(isOOM(r, b))
+                    r = nullptr;
+                return dist.distribute(r);
+            }
+
+            //! Wave free: every lane passes its pointer or nullptr.
+            MALLOCMC_D void free(void* pv) const
+            {
+                const u32 me = hw::laneid();
+                const u32 P = h.pagesize;
+                char* p = static_cast<char*>(pv);
+                u32 page = 0u, cs = 0u, inpage = 0u;
+                bool simple = false;
+                if(p != nullptr)
+                {
+                    page = h.pageOf(static_cast<u64>(p - h.pages));
+                    cs = atom::load(h.pteChunksize(page));
+                    inpage = static_cast<u32>(p - h.pageData(page));
+                    const bool chunked = cs != 0u
+                        && (cs < P || (cs == P && (h.flags & kFixPagesizeLeak)));
+                    simple = chunked && (inpage % cs) == 0u;
+                    if(!simple)
+                        h.destroy<WaveIds>(p);
+                }
+                u64 pending = hw::ballot(simple);
+                while(pending != 0u)
+                {
+                    const u32 leader = hw::ffs(pending) - 1u;
+                    const u32 lpage = wave::lane(page, leader);
+                    const bool member = ((pending >> me) & 1u) != 0u && page == lpage;
+                    const u64 group = hw::ballot(member);
+                    const u32 k = hw::popc(group);
+                    const u32 c = wave::lane(cs, leader);
+                    const u32 idx = member ? inpage / c : 0u;
+                    const bool hier = c <= h.hierarchyThreshold;
+                    if(hier)
+                    {
+                        const u32 segmentsize = c * 32u + 4u;
+                        const u32 full = umin32(32u, P / segmentsize);
+                        const u32 additional = h.calcAdditional(full, segmentsize, c);
+                        const u32 nMasks = full + (additional > 0u ? 1u : 0u);
+                        u32* masks = h.onPageMasks(lpage, nMasks);
+                        const u32 seg = idx / 32u;
+                        // combine the bits of each segment; the lowest lane of a segment clears them
+                        u64 segPending = group;
+                        u32 segMask = 0u;
+                        while(segPending != 0u)
+                        {
+                            const u32 l = hw::ffs(segPending) - 1u;
+                            const u32 s = wave::lane(seg, l);
+                            const bool inSeg = member && seg == s;
+                            const u32 bits = wave::bor(inSeg ? (1u << (idx % 32u)) : 0u);
+                            if(me == l)
+                                atom::and_(&masks[s], ~bits);
+                            segPending &= ~hw::ballot(inSeg);
+                            segMask |= 1u << s;
+                        }
+                        if(me == leader)
+                            atom::and_(h.pteBitmask(lpage), ~segMask);
+                    }
+                    else
+                    {
+                        const u32 bits = wave::bor(member ? (1u << idx) : 0u);
+                        if(me == leader)
+                            atom::and_(h.pteBitmask(lpage), ~bits);
+                    }
+                    u32 old = 0u;
+                    if(me == leader)
+                        old = atom::sub(h.pteCount(lpage), k);
+                    old = wave::lane(old, leader);
+                    // D4 for k chunks: the wave that empties the page resets it
+                    if(h.resetfreedpages && old == k)
+                    {
+                        u32 locked = 1u;
+                        if(me == leader)
+                            locked = atom::cas(h.pteCount(lpage), 0u, P);
+                        locked = wave::lane(locked, leader);
+                        if(locked == 0u)
+                        {
+                            u32* w = reinterpret_cast<u32*>(h.pageData(lpage) + P - 4u * h.maxOnPageMasks);
+                            if(me < h.maxOnPageMasks)
+                                atom::store(&w[me], 0u);
+                            atom::fence_device();
+                            if(me == leader)
+                            {
+                                atom::cas(h.pteChunksize(lpage), c, 0u);
+                                atom::fence_device();
+                                atom::sub(h.pteCount(lpage), P);
+                            }
+                        }
+                    }
+                    // region reset when the count passes pagesize/2/c (Scatter.hpp:727-735)
+                    const u32 thr = P / 2u / c;
+                    if(old >= thr && old - k < thr && me == leader)
+                    {
+                        atom::exch(&h.regions[lpage / h.regionsize], 0u);
+                        atom::min(&h.scalars->firstfreeblock, lpage / h.pagesperblock);
+                    }
+                    pending &= ~group;
+                }
+            }
+        };
+    } // namespace detail
+} // namespace mallocMC

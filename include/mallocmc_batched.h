@@ -1,0 +1,83 @@
+/*
+ * Batched entry points of libmallocmc_hip.so: wave-cooperative malloc / free
+ * (include/mallocMC/detail/wave_alloc.hpp), a persistent multi-heap step
+ * kernel that serves many independent heaps in one launch, and the
+ * device-resident serving loop.  They have no one-to-one counterpart in the
+ * reference, whose only device entry points are per-thread
+ * DeviceAllocator::malloc / free (src/include/mallocMC/device_allocator.hpp:72-88);
+ * they produce heap states those entry points can keep using.
+ * Conventions as in mallocmc_hip.h.
+ */
+#ifndef MALLOCMC_BATCHED_H
+#define MALLOCMC_BATCHED_H
+
+#include "mallocmc_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Tuning of the wave path. */
+typedef struct mmc_tuning {
+    uint32_t aggregation;  /* lanes served by one group claim: 1..64 (1 = per lane) */
+    uint32_t xcd_slices;   /* 0/1: reference hash; 8: one slice of each access block per XCD */
+    uint32_t persistent;   /* 0: one lane per request; 1: persistent waves pull 64-request tiles */
+    uint32_t waves_per_cu; /* persistent launch size (0 = 8) */
+} mmc_tuning;
+
+int mmc_default_tuning(mmc_tuning* t);
+
+/* Wave-cooperative counterpart of mmc_malloc_batch: same inputs and outputs,
+ * same heap protocol, lanes of a wavefront grouped by size class. */
+int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_offsets, uint64_t n,
+                          const mmc_tuning* tuning, void* stream);
+
+/* Wave-cooperative counterpart of mmc_free_batch (frees grouped by page). */
+int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, const mmc_tuning* tuning,
+                        void* stream);
+
+/* ---------------------------------------------------------------- multi-heap */
+typedef struct mmc_multi mmc_multi;
+
+/* Groups `nheaps` existing heaps for one-launch service; the heaps stay owned
+ * by the caller and must outlive the group. */
+int mmc_multi_create(mmc_heap* const* heaps, uint32_t nheaps, mmc_multi** out);
+int mmc_multi_destroy(mmc_multi* m);
+
+/* Work description of one step over many heaps.  Slot s (0 <= s < nslots)
+ * belongs to heap d_slot_heap[s]; slots of one heap are contiguous and each
+ * 64-slot tile of d_tiles (uint32 first slot, tile i covers
+ * [d_tiles[i], min(d_tiles[i] + 64, end of that heap's slots))) lies in one
+ * heap.  Each step, for every slot: a live slot (d_offsets[s] != MMC_NULL)
+ * frees its allocation with probability p_free_permille/1000; a dead slot
+ * allocates with probability p_alloc_permille/1000 a size drawn from the
+ * table d_size_table (n_sizes entries) with the cumulative weights
+ * d_size_cdf (uint32, last entry 2^32-1).  Randomness is splitmix64 of
+ * (seed, *d_step, slot).  *d_step is incremented by the launch, so a captured
+ * graph replays successive steps.  d_counters (uint64[4], accumulated):
+ * mallocs, frees, nulls, tiles processed. */
+typedef struct mmc_step_desc {
+    const uint32_t* d_slot_heap;
+    const uint32_t* d_tiles;
+    uint32_t n_tiles;
+    uint32_t n_slots;
+    const uint32_t* d_size_table;
+    const uint32_t* d_size_cdf;
+    uint32_t n_sizes;
+    uint32_t p_free_permille;
+    uint32_t p_alloc_permille;
+    uint32_t reserved;
+    uint64_t seed;
+    uint64_t* d_offsets;
+    uint32_t* d_sizes;    /* size of the live allocation per slot (for verification) */
+    uint64_t* d_step;
+    uint64_t* d_counters;
+    uint32_t* d_tile_cursor; /* work-queue cursor, reset by the launch itself */
+} mmc_step_desc;
+
+int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tuning, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -36,6 +36,53 @@ EXPORTED_SYMBOLS = [
     "mmc_write_pattern", "mmc_verify_pattern", "mmc_alloc_until_null", "mmc_bench_atomics",
     "mmc_bench_copy",
 ]
+BATCHED_SYMBOLS = [
+    "mmc_default_tuning", "mmc_wave_malloc_batch", "mmc_wave_free_batch", "mmc_multi_create",
+    "mmc_multi_destroy", "mmc_multi_step",
+]
+
+
+class MmcTuning(ctypes.Structure):
+    _fields_ = [
+        ("aggregation", ctypes.c_uint32),
+        ("xcd_slices", ctypes.c_uint32),
+        ("persistent", ctypes.c_uint32),
+        ("waves_per_cu", ctypes.c_uint32),
+    ]
+
+
+@dataclasses.dataclass
+class Tuning:
+    """Wave-path tuning (mmc_tuning): aggregation width 1..64, XCD slices,
+    persistent launch and its waves per CU."""
+    aggregation: int = 64
+    xcd_slices: int = 0
+    persistent: bool = False
+    waves_per_cu: int = 8
+
+    def to_c(self) -> MmcTuning:
+        return MmcTuning(self.aggregation, self.xcd_slices, int(bool(self.persistent)), self.waves_per_cu)
+
+
+class MmcStepDesc(ctypes.Structure):
+    _fields_ = [
+        ("d_slot_heap", ctypes.c_void_p),
+        ("d_tiles", ctypes.c_void_p),
+        ("n_tiles", ctypes.c_uint32),
+        ("n_slots", ctypes.c_uint32),
+        ("d_size_table", ctypes.c_void_p),
+        ("d_size_cdf", ctypes.c_void_p),
+        ("n_sizes", ctypes.c_uint32),
+        ("p_free_permille", ctypes.c_uint32),
+        ("p_alloc_permille", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+        ("d_offsets", ctypes.c_void_p),
+        ("d_sizes", ctypes.c_void_p),
+        ("d_step", ctypes.c_void_p),
+        ("d_counters", ctypes.c_void_p),
+        ("d_tile_cursor", ctypes.c_void_p),
+    ]
 
 
 class MmcConfig(ctypes.Structure):
@@ -155,6 +202,12 @@ def lib() -> ctypes.CDLL:
         "mmc_alloc_until_null": ([vp, u32, vp, u64, vp, u32, u32, vp], i32),
         "mmc_bench_atomics": ([vp, u64, u32, u32, u32, u32, vp, vp], i32),
         "mmc_bench_copy": ([vp, vp, u64, vp], i32),
+        "mmc_default_tuning": ([P(MmcTuning)], i32),
+        "mmc_wave_malloc_batch": ([vp, vp, vp, u64, P(MmcTuning), vp], i32),
+        "mmc_wave_free_batch": ([vp, vp, u64, P(MmcTuning), vp], i32),
+        "mmc_multi_create": ([P(vp), u32, P(vp)], i32),
+        "mmc_multi_destroy": ([vp], i32),
+        "mmc_multi_step": ([vp, P(MmcStepDesc), P(MmcTuning), vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -274,6 +327,23 @@ class ScatterAllocator:
         _require_cuda(offsets, "offsets")
         _check(lib().mmc_free_batch(self.handle, _ptr(offsets), offsets.numel(), _stream_ptr(stream)), "free_batch")
 
+    def wave_malloc_batch(self, sizes, offsets=None, tuning: Tuning | None = None, stream=None):
+        """Wave-cooperative malloc of sizes[i] (same contract as malloc_batch)."""
+        import torch
+        _require_cuda(sizes, "sizes")
+        if offsets is None:
+            offsets = torch.empty(sizes.numel(), dtype=torch.int64, device=sizes.device)
+        t = (tuning or Tuning()).to_c()
+        _check(lib().mmc_wave_malloc_batch(self.handle, _ptr(sizes), _ptr(offsets), sizes.numel(), ctypes.byref(t),
+                                           _stream_ptr(stream)), "wave_malloc_batch")
+        return offsets
+
+    def wave_free_batch(self, offsets, tuning: Tuning | None = None, stream=None):
+        _require_cuda(offsets, "offsets")
+        t = (tuning or Tuning()).to_c()
+        _check(lib().mmc_wave_free_batch(self.handle, _ptr(offsets), offsets.numel(), ctypes.byref(t),
+                                         _stream_ptr(stream)), "wave_free_batch")
+
     def serial_stream(self, ops, stream=None):
         """Serialized single-lane stream with the CPU backend's ids (see C ABI)."""
         import torch
@@ -305,6 +375,33 @@ class ScatterAllocator:
                                           _stream_ptr(stream)), "alloc_until_null")
         n = int(cnt.item())
         return offs[: min(n, capacity)], n
+
+
+class MultiHeap:
+    """Many independent heaps served by one persistent launch per step
+    (mmc_multi_*).  The heaps stay owned by their ScatterAllocator objects."""
+
+    def __init__(self, allocators):
+        self.allocators = list(allocators)
+        arr = (ctypes.c_void_p * len(self.allocators))(*[a.handle.value for a in self.allocators])
+        self._m = ctypes.c_void_p()
+        _check(lib().mmc_multi_create(arr, len(self.allocators), ctypes.byref(self._m)), "mmc_multi_create")
+
+    def close(self):
+        if self._m:
+            lib().mmc_multi_destroy(self._m)
+            self._m = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, desc: MmcStepDesc, tuning: Tuning | None = None, stream=None):
+        t = (tuning or Tuning(persistent=True)).to_c()
+        _check(lib().mmc_multi_step(self._m, ctypes.byref(desc), ctypes.byref(t), _stream_ptr(stream)),
+               "mmc_multi_step")
 
 
 def default_config() -> ScatterConfig:

@@ -1,0 +1,353 @@
+// Batched entry points: wave-cooperative malloc/free over one heap (plain or
+// persistent launch) and the persistent multi-heap step kernel.
+// Declarations: include/mallocmc_batched.h.
+#include "mallocmc_batched.h"
+
+#include "mallocMC/detail/device_allocator.hpp"
+#include "mallocMC/detail/wave_alloc.hpp"
+#include "mmc_internal.hpp"
+
+#include <vector>
+
+using namespace mallocMC;
+using namespace mallocMC::detail;
+
+struct mmc_multi
+{
+    ScatterHeap* dHeaps = nullptr;
+    std::uint32_t nheaps = 0;
+    std::uint32_t distribution = MMC_DIST_NOOP;
+    int device = 0;
+};
+
+namespace
+{
+    WaveTuning toTuning(const mmc_tuning* t)
+    {
+        WaveTuning w;
+        if(t)
+        {
+            w.aggregation = t->aggregation == 0u ? 64u : (t->aggregation > 64u ? 64u : t->aggregation);
+            w.xcdSlices = t->xcd_slices;
+        }
+        return w;
+    }
+
+    //! Lane i serves request i; the whole wave cooperates.
+    template<class Dist>
+    __global__ __launch_bounds__(256) void waveMallocKernel(ScatterHeap h, WaveTuning t, const u32* sizes,
+                                                            u64* offsets, u64 n)
+    {
+        const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+        const u32 bytes = i < n ? sizes[i] : 0u;
+        const WaveAlloc wa{h, t};
+        void* p = wa.malloc<Dist>(bytes);
+        if(i < n)
+            offsets[i] = h.offsetOf(p);
+    }
+
+    __global__ __launch_bounds__(256) void waveFreeKernel(ScatterHeap h, WaveTuning t, const u64* offsets, u64 n)
+    {
+        const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+        const u64 off = i < n ? offsets[i] : kNullOffset;
+        const WaveAlloc wa{h, t};
+        wa.free(off == kNullOffset ? nullptr : h.pages + off);
+    }
+
+    //! Work-queue helpers shared by the persistent kernels: cursor[0] hands out
+    //! tiles, cursor[1] counts finished waves; the last wave to finish resets
+    //! both so that the next launch (or graph replay) starts from zero.
+    MALLOCMC_D u32 nextTile(u32* cursor)
+    {
+        u32 tile = 0u;
+        if(hw::laneid() == 0u)
+            tile = atom::add(&cursor[0], 1u);
+        return __builtin_amdgcn_readfirstlane(tile);
+    }
+
+    MALLOCMC_D bool finishWave(u32* cursor, u32 totalWaves)
+    {
+        u32 done = 0u;
+        if(hw::laneid() == 0u)
+            done = atom::add(&cursor[1], 1u);
+        done = __builtin_amdgcn_readfirstlane(done);
+        if(done == totalWaves - 1u)
+        {
+            if(hw::laneid() == 0u)
+            {
+                atom::store(&cursor[0], 0u);
+                atom::store(&cursor[1], 0u);
+            }
+            return true;
+        }
+        return false;
+    }
+
+    template<class Dist>
+    __global__ __launch_bounds__(256) void persistentMallocKernel(ScatterHeap h, WaveTuning t, const u32* sizes,
+                                                                  u64* offsets, u64 n, u32* cursor)
+    {
+        const u32 totalWaves = gridDim.x * (blockDim.x / 64u);
+        const u64 nTiles = (n + 63u) / 64u;
+        const WaveAlloc wa{h, t};
+        for(;;)
+        {
+            const u32 tile = nextTile(cursor);
+            if(tile >= nTiles)
+                break;
+            const u64 i = static_cast<u64>(tile) * 64u + hw::laneid();
+            const u32 bytes = i < n ? sizes[i] : 0u;
+            void* p = wa.malloc<Dist>(bytes);
+            if(i < n)
+                offsets[i] = h.offsetOf(p);
+        }
+        finishWave(cursor, totalWaves);
+    }
+
+    __global__ __launch_bounds__(256) void persistentFreeKernel(ScatterHeap h, WaveTuning t, const u64* offsets, u64 n,
+                                                                u32* cursor)
+    {
+        const u32 totalWaves = gridDim.x * (blockDim.x / 64u);
+        const u64 nTiles = (n + 63u) / 64u;
+        const WaveAlloc wa{h, t};
+        for(;;)
+        {
+            const u32 tile = nextTile(cursor);
+            if(tile >= nTiles)
+                break;
+            const u64 i = static_cast<u64>(tile) * 64u + hw::laneid();
+            const u64 off = i < n ? offsets[i] : kNullOffset;
+            wa.free(off == kNullOffset ? nullptr : h.pages + off);
+        }
+        finishWave(cursor, totalWaves);
+    }
+
+    MALLOCMC_D u64 mix64(u64 z)
+    {
+        z += 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+
+    //! One step over many heaps (see mmc_step_desc).
+    template<class Dist>
+    __global__ __launch_bounds__(256) void multiStepKernel(const ScatterHeap* heaps, mmc_step_desc d, WaveTuning t)
+    {
+        const u32 totalWaves = gridDim.x * (blockDim.x / 64u);
+        const u64 step = *d.d_step;
+        const u32 me = hw::laneid();
+        u64 nMalloc = 0, nFree = 0, nNull = 0, nTiles = 0;
+        for(;;)
+        {
+            const u32 tile = nextTile(d.d_tile_cursor);
+            if(tile >= d.n_tiles)
+                break;
+            ++nTiles;
+            const u32 first = d.d_tiles[tile];
+            const u32 heapIdx = d.d_slot_heap[first];
+            const u32 s = first + me;
+            const bool valid = s < d.n_slots && d.d_slot_heap[s] == heapIdx;
+            const ScatterHeap h = heaps[heapIdx];
+            const WaveAlloc wa{h, t};
+            const u64 r = mix64(d.seed ^ mix64(step * 0x100000001B3ull + s));
+            const u32 coin = static_cast<u32>(r % 1000u);
+            u64 off = valid ? d.d_offsets[s] : kNullOffset;
+            const bool live = off != kNullOffset;
+            const bool doFree = valid && live && coin < d.p_free_permille;
+            const bool doAlloc = valid && !live && coin < d.p_alloc_permille;
+            wa.free(doFree ? h.pages + off : nullptr);
+            u32 bytes = 0u;
+            if(doAlloc)
+            {
+                const u32 u = static_cast<u32>(r >> 32);
+                u32 k = 0u;
+                while(k + 1u < d.n_sizes && u > d.d_size_cdf[k])
+                    ++k;
+                bytes = d.d_size_table[k];
+            }
+            void* p = wa.malloc<Dist>(bytes);
+            if(doFree)
+            {
+                d.d_offsets[s] = kNullOffset;
+                d.d_sizes[s] = 0u;
+                ++nFree;
+            }
+            if(doAlloc)
+            {
+                ++nMalloc;
+                if(p == nullptr)
+                    ++nNull;
+                d.d_offsets[s] = h.offsetOf(p);
+                d.d_sizes[s] = p == nullptr ? 0u : bytes;
+            }
+        }
+        // wave totals into the counters
+        for(int x = 32; x > 0; x >>= 1)
+        {
+            nMalloc += __shfl_xor(nMalloc, x, 64);
+            nFree += __shfl_xor(nFree, x, 64);
+            nNull += __shfl_xor(nNull, x, 64);
+        }
+        if(me == 0u)
+        {
+            if(nMalloc)
+                atom::add(&d.d_counters[0], nMalloc);
+            if(nFree)
+                atom::add(&d.d_counters[1], nFree);
+            if(nNull)
+                atom::add(&d.d_counters[2], nNull);
+            if(nTiles)
+                atom::add(&d.d_counters[3], nTiles);
+        }
+        if(finishWave(d.d_tile_cursor, totalWaves) && me == 0u)
+            atom::add(d.d_step, u64{1});
+    }
+
+    unsigned persistentBlocks(const mmc_tuning* t)
+    {
+        int dev = 0, cus = 256;
+        if(hipGetDevice(&dev) == hipSuccess)
+            (void) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const unsigned wpc = (t && t->waves_per_cu) ? t->waves_per_cu : 8u;
+        const unsigned blocks = static_cast<unsigned>(cus) * wpc / 4u; // 4 waves per 256-thread block
+        return blocks ? blocks : 1u;
+    }
+} // namespace
+
+namespace mmc
+{
+    u32* heapCursor(mmc_heap* heap);
+}
+
+extern "C" {
+
+int mmc_default_tuning(mmc_tuning* t)
+{
+    if(!t)
+        return mmc::fail(MMC_EINVAL, "null tuning");
+    t->aggregation = 64u;
+    t->xcd_slices = 0u;
+    t->persistent = 0u;
+    t->waves_per_cu = 8u;
+    return MMC_OK;
+}
+
+int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_offsets, uint64_t n,
+                          const mmc_tuning* tuning, void* stream)
+{
+    if(!heap || (n && (!d_sizes || !d_offsets)))
+        return mmc::fail(MMC_EINVAL, "null argument");
+    if(n == 0)
+        return MMC_OK;
+    if(n > (1ull << 37))
+        return mmc::fail(MMC_EINVAL, "batch too large");
+    const WaveTuning t = toTuning(tuning);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool xm = heap->cfg.distribution == MMC_DIST_XMALLOCSIMD;
+    if(tuning && tuning->persistent)
+    {
+        const unsigned blocks = persistentBlocks(tuning);
+        u32* cur = mmc::heapCursor(heap);
+        if(xm)
+            hipLaunchKernelGGL(persistentMallocKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, heap->view, t,
+                               d_sizes, d_offsets, n, cur);
+        else
+            hipLaunchKernelGGL(persistentMallocKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, heap->view,
+                               t, d_sizes, d_offsets, n, cur);
+    }
+    else
+    {
+        const unsigned blocks = mmc::gridFor(n, 256);
+        if(xm)
+            hipLaunchKernelGGL(waveMallocKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, heap->view, t, d_sizes,
+                               d_offsets, n);
+        else
+            hipLaunchKernelGGL(waveMallocKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, heap->view, t,
+                               d_sizes, d_offsets, n);
+    }
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "wave malloc launch");
+}
+
+int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, const mmc_tuning* tuning,
+                        void* stream)
+{
+    if(!heap || (n && !d_offsets))
+        return mmc::fail(MMC_EINVAL, "null argument");
+    if(n == 0)
+        return MMC_OK;
+    const WaveTuning t = toTuning(tuning);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if(tuning && tuning->persistent)
+        hipLaunchKernelGGL(persistentFreeKernel, dim3(persistentBlocks(tuning)), dim3(256), 0, s, heap->view, t,
+                           d_offsets, n, mmc::heapCursor(heap));
+    else
+        hipLaunchKernelGGL(waveFreeKernel, dim3(mmc::gridFor(n, 256)), dim3(256), 0, s, heap->view, t, d_offsets, n);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "wave free launch");
+}
+
+int mmc_multi_create(mmc_heap* const* heaps, uint32_t nheaps, mmc_multi** out)
+{
+    if(!heaps || nheaps == 0 || !out)
+        return mmc::fail(MMC_EINVAL, "bad argument");
+    *out = nullptr;
+    std::vector<ScatterHeap> views(nheaps);
+    const u32 dist = heaps[0]->cfg.distribution;
+    for(u32 i = 0; i < nheaps; ++i)
+    {
+        if(!heaps[i])
+            return mmc::fail(MMC_EINVAL, "null heap in group");
+        if(heaps[i]->cfg.distribution != dist)
+            return mmc::fail(MMC_EINVAL, "heaps of one group must share the distribution policy");
+        views[i] = heaps[i]->view;
+    }
+    mmc_multi* m = new mmc_multi();
+    m->nheaps = nheaps;
+    m->distribution = dist;
+    hipError_t err = hipMalloc(&m->dHeaps, sizeof(ScatterHeap) * nheaps);
+    if(err == hipSuccess)
+        err = hipMemcpy(m->dHeaps, views.data(), sizeof(ScatterHeap) * nheaps, hipMemcpyHostToDevice);
+    if(err != hipSuccess)
+    {
+        if(m->dHeaps)
+            (void) hipFree(m->dHeaps);
+        delete m;
+        return mmc::hipFail(err, "multi-heap table");
+    }
+    *out = m;
+    return MMC_OK;
+}
+
+int mmc_multi_destroy(mmc_multi* m)
+{
+    if(!m)
+        return MMC_OK;
+    (void) hipDeviceSynchronize();
+    if(m->dHeaps)
+        (void) hipFree(m->dHeaps);
+    delete m;
+    return MMC_OK;
+}
+
+int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tuning, void* stream)
+{
+    if(!m || !desc || !desc->d_slot_heap || !desc->d_tiles || !desc->d_size_table || !desc->d_size_cdf
+       || !desc->d_offsets || !desc->d_sizes || !desc->d_step || !desc->d_counters || !desc->d_tile_cursor
+       || desc->n_sizes == 0)
+        return mmc::fail(MMC_EINVAL, "bad step descriptor");
+    if(desc->n_tiles == 0)
+        return MMC_OK;
+    const WaveTuning t = toTuning(tuning);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned blocks = persistentBlocks(tuning);
+    if(m->distribution == MMC_DIST_XMALLOCSIMD)
+        hipLaunchKernelGGL(multiStepKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc, t);
+    else
+        hipLaunchKernelGGL(multiStepKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc, t);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "multi step launch");
+}
+
+} // extern "C"

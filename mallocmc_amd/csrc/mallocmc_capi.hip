@@ -273,6 +273,14 @@ namespace mmc
             heap->rawPool = nullptr;
             return hipFail(err, "hipMalloc(heap metadata)");
         }
+        // slot-count output and the persistent-kernel work cursor start at zero
+        err = hipMemsetAsync(heap->scratch + 4u * kRunThreads, 0, sizeof(u32) * 64u, stream);
+        if(err != hipSuccess)
+        {
+            (void) hipFree(raw);
+            heap->rawPool = nullptr;
+            return hipFail(err, "hipMemsetAsync(heap scratch)");
+        }
         heap->pool = pool;
         heap->memsize = memsize;
         heap->geom = g;
@@ -330,6 +338,12 @@ namespace mmc
         if(err != hipSuccess)
             return hipFail(err, "slots kernel launch");
         return MMC_OK;
+    }
+
+    //! Work cursor of the persistent kernels (two words, self-resetting).
+    u32* heapCursor(mmc_heap* heap)
+    {
+        return heap->scratch + 4u * kRunThreads + 16u;
     }
 
     std::string infoString(const mmc_heap* heap)
