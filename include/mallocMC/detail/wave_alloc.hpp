@@ -338,14 +338,20 @@ namespace mallocMC
                                     else if(pcs >= bmax && pcs <= maxcs && pcs <= h.pagesize)
                                         cand = pcount < h.pageCapacity(pcs);
                                 }
-                                u64 cands = hw::ballot(cand);
+                                // partially used pages of a fitting chunk size first, free
+                                // pages after them: formatting a fresh page while a fitting
+                                // one has room is what fragments the reference's heap
+                                u64 used = hw::ballot(cand && pcs != 0u);
+                                u64 cands = hw::ballot(cand && pcs == 0u);
                                 // cyclic order from the hashed page within the region
                                 const u32 rot = (gp - regionOffset) >= base ? (gp - regionOffset) - base : 0u;
-                                while(cands != 0u && need != 0u)
+                                while((used | cands) != 0u && need != 0u)
                                 {
-                                    const u64 hi = rot < 64u ? (cands >> rot) << rot : 0u;
-                                    const u64 pick = hi != 0u ? hi : cands;
+                                    const u64 set = used != 0u ? used : cands;
+                                    const u64 hi = rot < 64u ? (set >> rot) << rot : 0u;
+                                    const u64 pick = hi != 0u ? hi : set;
                                     const u32 l = hw::ffs(pick) - 1u;
+                                    used &= ~(1ull << l);
                                     cands &= ~(1ull << l);
                                     const u32 page = regionOffset + base + l;
                                     u32 cs = wave::lane(pcs, l);
@@ -439,7 +445,11 @@ namespace mallocMC
                 return dist.distribute(r);
             }
 
-            //! Wave free: every lane passes its pointer or nullptr.
+            //! Wave free: every lane passes its pointer or nullptr.  Lanes are
+            //! grouped by page (and by 32-chunk segment on hierarchical pages)
+            //! in registers first; then the leaders of all groups issue their
+            //! atomics in the same vector instructions, so a wave pays a few
+            //! round trips whatever the number of pages it touches.
             MALLOCMC_D void free(void* pv) const
             {
                 const u32 me = hw::laneid();
@@ -456,83 +466,98 @@ namespace mallocMC
                         && (cs < P || (cs == P && (h.flags & kFixPagesizeLeak)));
                     simple = chunked && (inpage % cs) == 0u;
                     if(!simple)
-                        h.destroy<WaveIds>(p);
+                        h.destroy<WaveIds>(p); // multi-page, XMallocSIMD slice, stale pointer
                 }
+                const u32 idx = simple ? inpage / cs : 0u;
+                const bool hier = simple && cs <= h.hierarchyThreshold;
+                const u32 seg = hier ? idx / 32u : 0u;
+
+                // ---- grouping (registers only)
+                bool pageLeader = false, segLeader = false;
+                u32 pageK = 0u, pageBits = 0u, segBits = 0u;
                 u64 pending = hw::ballot(simple);
                 while(pending != 0u)
                 {
                     const u32 leader = hw::ffs(pending) - 1u;
                     const u32 lpage = wave::lane(page, leader);
-                    const bool member = ((pending >> me) & 1u) != 0u && page == lpage;
+                    const bool member = simple && page == lpage;
                     const u64 group = hw::ballot(member);
-                    const u32 k = hw::popc(group);
-                    const u32 c = wave::lane(cs, leader);
-                    const u32 idx = member ? inpage / c : 0u;
-                    const bool hier = c <= h.hierarchyThreshold;
-                    if(hier)
+                    // PTE bitmask bits to clear: chunk bits (flat) or segment hints (hierarchical)
+                    const u32 bits = wave::bor(member ? (hier ? (1u << seg) : (1u << idx)) : 0u);
+                    if(me == leader)
                     {
-                        const u32 segmentsize = c * 32u + 4u;
-                        const u32 full = umin32(32u, P / segmentsize);
-                        const u32 additional = h.calcAdditional(full, segmentsize, c);
-                        const u32 nMasks = full + (additional > 0u ? 1u : 0u);
-                        u32* masks = h.onPageMasks(lpage, nMasks);
-                        const u32 seg = idx / 32u;
-                        // combine the bits of each segment; the lowest lane of a segment clears them
+                        pageLeader = true;
+                        pageK = hw::popc(group);
+                        pageBits = bits;
+                    }
+                    if(wave::lane(static_cast<u32>(hier), leader) != 0u)
+                    {
                         u64 segPending = group;
-                        u32 segMask = 0u;
                         while(segPending != 0u)
                         {
                             const u32 l = hw::ffs(segPending) - 1u;
                             const u32 s = wave::lane(seg, l);
                             const bool inSeg = member && seg == s;
-                            const u32 bits = wave::bor(inSeg ? (1u << (idx % 32u)) : 0u);
+                            const u32 b = wave::bor(inSeg ? (1u << (idx % 32u)) : 0u);
                             if(me == l)
-                                atom::and_(&masks[s], ~bits);
-                            segPending &= ~hw::ballot(inSeg);
-                            segMask |= 1u << s;
-                        }
-                        if(me == leader)
-                            atom::and_(h.pteBitmask(lpage), ~segMask);
-                    }
-                    else
-                    {
-                        const u32 bits = wave::bor(member ? (1u << idx) : 0u);
-                        if(me == leader)
-                            atom::and_(h.pteBitmask(lpage), ~bits);
-                    }
-                    u32 old = 0u;
-                    if(me == leader)
-                        old = atom::sub(h.pteCount(lpage), k);
-                    old = wave::lane(old, leader);
-                    // D4 for k chunks: the wave that empties the page resets it
-                    if(h.resetfreedpages && old == k)
-                    {
-                        u32 locked = 1u;
-                        if(me == leader)
-                            locked = atom::cas(h.pteCount(lpage), 0u, P);
-                        locked = wave::lane(locked, leader);
-                        if(locked == 0u)
-                        {
-                            u32* w = reinterpret_cast<u32*>(h.pageData(lpage) + P - 4u * h.maxOnPageMasks);
-                            if(me < h.maxOnPageMasks)
-                                atom::store(&w[me], 0u);
-                            atom::fence_device();
-                            if(me == leader)
                             {
-                                atom::cas(h.pteChunksize(lpage), c, 0u);
-                                atom::fence_device();
-                                atom::sub(h.pteCount(lpage), P);
+                                segLeader = true;
+                                segBits = b;
                             }
+                            segPending &= ~hw::ballot(inSeg);
                         }
-                    }
-                    // region reset when the count passes pagesize/2/c (Scatter.hpp:727-735)
-                    const u32 thr = P / 2u / c;
-                    if(old >= thr && old - k < thr && me == leader)
-                    {
-                        atom::exch(&h.regions[lpage / h.regionsize], 0u);
-                        atom::min(&h.scalars->firstfreeblock, lpage / h.pagesperblock);
                     }
                     pending &= ~group;
+                }
+
+                // ---- clear the chunk bits: every group at once
+                if(segLeader)
+                {
+                    const u32 segmentsize = cs * 32u + 4u;
+                    const u32 full = umin32(32u, P / segmentsize);
+                    const u32 additional = h.calcAdditional(full, segmentsize, cs);
+                    u32* masks = h.onPageMasks(page, full + (additional > 0u ? 1u : 0u));
+                    atom::and_(&masks[seg], ~segBits);
+                }
+                u32 old = 0u;
+                if(pageLeader)
+                {
+                    atom::and_(h.pteBitmask(page), ~pageBits);
+                    old = atom::sub(h.pteCount(page), pageK);
+                    // region reset when the count passes pagesize/2/c (Scatter.hpp:727-735),
+                    // at least when the page empties: for c > pagesize/2 the reference's
+                    // threshold is 0 and never fires, so failed scans ratchet every
+                    // region to "full" and each later request scans the whole heap.
+                    // Test first so hot words are only written when they change.
+                    const u32 thr = umax32(P / 2u / cs, 1u);
+                    if(old >= thr && old - pageK < thr)
+                    {
+                        u32* region = &h.regions[page / h.regionsize];
+                        if(atom::load(region) != 0u)
+                            atom::exch(region, 0u);
+                        const u32 block = page / h.pagesperblock;
+                        if(block < atom::load(&h.scalars->firstfreeblock))
+                            atom::min(&h.scalars->firstfreeblock, block);
+                    }
+                }
+                // ---- D4 for k chunks: the group that empties its page resets it
+                if(!h.resetfreedpages)
+                    return;
+                bool locked = false;
+                if(pageLeader && old == pageK)
+                    locked = atom::cas(h.pteCount(page), 0u, P) == 0u;
+                if(hw::ballot(locked) == 0u)
+                    return;
+                // order as Scatter.hpp:691-710: clear masks, drop the chunk size,
+                // unlock (the count lock keeps allocators out meanwhile).  The
+                // reference's full device fence before the unlock is a release:
+                // the unlock carries it, which drops the L2 invalidate a
+                // sequentially consistent fence adds on gfx950.
+                if(locked)
+                {
+                    h.pageInit(page);
+                    atom::cas(h.pteChunksize(page), cs, 0u);
+                    atom::sub_release(h.pteCount(page), P);
                 }
             }
         };

@@ -145,6 +145,13 @@ namespace mallocMC
         {
             return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        //! Release-ordered subtract: every store of this lane before it is
+        //! visible to any agent lane that observes its result.
+        template<class T>
+        MALLOCMC_D T sub_release(T* p, T v)
+        {
+            return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
         //! Compare-and-swap returning the old value (CUDA atomicCAS contract).
         template<class T>
         MALLOCMC_D T cas(T* p, T cmp, T val)

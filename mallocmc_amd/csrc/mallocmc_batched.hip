@@ -204,11 +204,27 @@ namespace
             atom::add(d.d_step, u64{1});
     }
 
+    //! CU count of the current device, queried once per device (never inside
+    //! a stream capture after the first call).
+    int cuCount()
+    {
+        static int cached[64] = {0};
+        int dev = 0;
+        if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+            return 256;
+        if(cached[dev] == 0)
+        {
+            int cus = 256;
+            if(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+                cus = 256;
+            cached[dev] = cus;
+        }
+        return cached[dev];
+    }
+
     unsigned persistentBlocks(const mmc_tuning* t)
     {
-        int dev = 0, cus = 256;
-        if(hipGetDevice(&dev) == hipSuccess)
-            (void) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const int cus = cuCount();
         const unsigned wpc = (t && t->waves_per_cu) ? t->waves_per_cu : 8u;
         const unsigned blocks = static_cast<unsigned>(cus) * wpc / 4u; // 4 waves per 256-thread block
         return blocks ? blocks : 1u;
