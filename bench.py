@@ -227,6 +227,12 @@ def main():
     alloc.close()
     del graphs
     mal_avg_ms = float(np.mean(mal_ms[1:]))
+    # HBM bytes per malloc launch from the committed counter passes (see the file)
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "batched_config2_hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("waveMallocKernel<NoopDistribution>", {}).get("hbm_bytes_per_launch")
     achieved_gbs = N_LANES * MALLOC_BYTES_PER_REQUEST / (mal_avg_ms / 1e3) / 1e9
     atom_rate = N_LANES * MALLOC_RMW_PER_REQUEST / (mal_avg_ms / 1e3)
 
@@ -287,7 +293,7 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": {
             "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "waveMallocKernel", "avg_launch_ms": mal_avg_ms,
             "bytes_per_request": MALLOC_BYTES_PER_REQUEST,
         },

@@ -88,6 +88,13 @@ int mmc_compute_geometry(const mmc_config* cfg, uint64_t bytes, mmc_geometry* ou
  * init kernel (Scatter.hpp:1057-1096) on `stream`. */
 int mmc_create(const mmc_config* cfg, uint64_t bytes, void* stream, mmc_heap** out);
 
+/* Heap inside a pool the caller reserved (ReservePool policy,
+ * reservePoolPolicies/AlpakaBuf.hpp:41-46, then AlignmentPolicy::alignPool,
+ * Shrink.hpp:109-130): `pool` must already be aligned to cfg->dataAlignment.
+ * Enqueues the init kernel (Scatter.hpp:1057-1096) on `stream`.  The heap does
+ * not own the pool: mmc_destroy leaves it to the caller. */
+int mmc_create_in_pool(const mmc_config* cfg, void* pool, uint64_t bytes, void* stream, mmc_heap** out);
+
 /* Allocator dtor (allocator.hpp:185-189).  Synchronizes the device first. */
 int mmc_destroy(mmc_heap* heap);
 

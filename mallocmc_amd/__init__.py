@@ -30,6 +30,7 @@ FLAG_FIX_PAGESIZE_LEAK = 1
 # include/mallocmc_batched.h).
 EXPORTED_SYMBOLS = [
     "mmc_last_error", "mmc_version", "mmc_default_config", "mmc_compute_geometry", "mmc_create",
+    "mmc_create_in_pool",
     "mmc_destroy", "mmc_destructive_resize", "mmc_heap_location", "mmc_get_config", "mmc_get_geometry",
     "mmc_info", "mmc_available_slots", "mmc_available_slots_async", "mmc_get_handle",
     "mmc_malloc_batch", "mmc_free_batch", "mmc_serial_stream",
@@ -185,6 +186,7 @@ def lib() -> ctypes.CDLL:
         "mmc_default_config": ([P(MmcConfig)], i32),
         "mmc_compute_geometry": ([P(MmcConfig), u64, P(MmcGeometry)], i32),
         "mmc_create": ([P(MmcConfig), u64, vp, P(vp)], i32),
+        "mmc_create_in_pool": ([P(MmcConfig), vp, u64, vp, P(vp)], i32),
         "mmc_destroy": ([vp], i32),
         "mmc_destructive_resize": ([vp, u64, vp], i32),
         "mmc_heap_location": ([vp, P(vp), P(u64)], i32),

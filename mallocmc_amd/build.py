@@ -65,6 +65,30 @@ def build(verbose: bool = False) -> str:
     return LIB_PATH
 
 
+CPP_DIR = os.path.join(REPO_DIR, "tests", "cpp")
+CPP_PROGRAMS = ["example_arrays", "example_slots", "verify_heap"]
+
+
+def build_cpp_programs(verbose: bool = False) -> list:
+    """C++ programs written against the drop-in headers (include/mallocMC),
+    linked with libmallocmc_hip.so; outputs in tests/cpp/_build/."""
+    cc = hipcc()
+    out_dir = os.path.join(CPP_DIR, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    outs = []
+    for prog in CPP_PROGRAMS:
+        src = os.path.join(CPP_DIR, prog + ".hip")
+        out = os.path.join(out_dir, prog)
+        cmd = [cc, *[f for f in HIPCC_FLAGS if f != "-fPIC"], "-I", INCLUDE, src, "-o", out + ".tmp",
+               "-L", PKG_DIR, "-lmallocmc_hip", "-Wl,-rpath," + PKG_DIR + ",-rpath,$ORIGIN/../../../mallocmc_amd"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
+
+
 def build_oracle(verbose: bool = False) -> None:
     oracle_dir = os.path.join(REPO_DIR, "oracle")
     cmd = ["make", "-s", "-C", oracle_dir]
@@ -77,3 +101,4 @@ if __name__ == "__main__":
     v = "--verbose" in sys.argv
     build_oracle(v)
     print(build(v))
+    print(build_cpp_programs(v))

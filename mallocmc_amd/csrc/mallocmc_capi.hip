@@ -234,7 +234,9 @@ namespace mmc
 {
     constexpr u32 kRunThreads = 16384u;
 
-    int initHeap(mmc_heap* heap, u64 bytes, hipStream_t stream)
+    //! `external`: a caller-owned, already aligned pool; else the library
+    //! reserves one (AlpakaBuf.hpp:41-46) and aligns it (Shrink.hpp:109-130).
+    int initHeap(mmc_heap* heap, u64 bytes, hipStream_t stream, void* external = nullptr)
     {
         const ScatterConfig cfg = toConfig(heap->cfg);
         if(const char* e = validateConfig(cfg))
@@ -242,25 +244,36 @@ namespace mmc
         hipError_t err = hipGetDevice(&heap->device);
         if(err != hipSuccess)
             return hipFail(err, "hipGetDevice");
-        // ReservePool (AlpakaBuf.hpp:41-46) then Shrink::alignPool (Shrink.hpp:109-130)
-        void* raw = nullptr;
-        err = hipMalloc(&raw, bytes);
-        if(err != hipSuccess)
-            return hipFail(err, "hipMalloc(pool)");
-        heap->rawPool = raw;
-        heap->rawBytes = bytes;
+        void* raw = external;
         u64 memsize = bytes;
         char* pool = static_cast<char*>(raw);
-        const u64 misalign = reinterpret_cast<u64>(pool) & (cfg.dataAlignment - 1u);
-        if(misalign != 0u)
+        if(external)
         {
-            pool += cfg.dataAlignment - misalign;
-            memsize -= cfg.dataAlignment + misalign;
+            if((reinterpret_cast<u64>(external) & (cfg.dataAlignment - 1u)) != 0u)
+                return fail(MMC_EINVAL, "pool is not aligned to dataAlignment");
+            heap->ownsPool = false;
         }
+        else
+        {
+            err = hipMalloc(&raw, bytes);
+            if(err != hipSuccess)
+                return hipFail(err, "hipMalloc(pool)");
+            heap->ownsPool = true;
+            pool = static_cast<char*>(raw);
+            const u64 misalign = reinterpret_cast<u64>(pool) & (cfg.dataAlignment - 1u);
+            if(misalign != 0u)
+            {
+                pool += cfg.dataAlignment - misalign;
+                memsize -= cfg.dataAlignment + misalign;
+            }
+        }
+        heap->rawPool = heap->ownsPool ? raw : nullptr;
+        heap->rawBytes = bytes;
         const ScatterGeometry g = computeGeometry(cfg, memsize);
         if(g.numpages == 0u)
         {
-            (void) hipFree(raw);
+            if(heap->ownsPool)
+                (void) hipFree(raw);
             heap->rawPool = nullptr;
             return fail(MMC_EINVAL, "heap too small: the geometry has zero pages");
         }
@@ -269,7 +282,8 @@ namespace mmc
             err = hipMalloc(&heap->scratch, sizeof(u32) * (4u * kRunThreads + 64u));
         if(err != hipSuccess)
         {
-            (void) hipFree(raw);
+            if(heap->ownsPool)
+                (void) hipFree(raw);
             heap->rawPool = nullptr;
             return hipFail(err, "hipMalloc(heap metadata)");
         }
@@ -277,7 +291,8 @@ namespace mmc
         err = hipMemsetAsync(heap->scratch + 4u * kRunThreads, 0, sizeof(u32) * 64u, stream);
         if(err != hipSuccess)
         {
-            (void) hipFree(raw);
+            if(heap->ownsPool)
+                (void) hipFree(raw);
             heap->rawPool = nullptr;
             return hipFail(err, "hipMemsetAsync(heap scratch)");
         }
@@ -439,6 +454,28 @@ int mmc_create(const mmc_config* cfg, uint64_t bytes, void* stream, mmc_heap** o
     return MMC_OK;
 }
 
+int mmc_create_in_pool(const mmc_config* cfg, void* pool, uint64_t bytes, void* stream, mmc_heap** out)
+{
+    if(!cfg || !out || (!pool && bytes != 0))
+        return fail(MMC_EINVAL, "Scatter policy cannot use nullptr for non-empty memory pools");
+    *out = nullptr;
+    if(cfg->distribution > MMC_DIST_XMALLOCSIMD)
+        return fail(MMC_EINVAL, "unknown distribution policy");
+    mmc_heap* heap = new(std::nothrow) mmc_heap();
+    if(!heap)
+        return fail(MMC_ENOMEM, "host allocation failed");
+    heap->cfg = *cfg;
+    const int rc = mmc::initHeap(heap, bytes, static_cast<hipStream_t>(stream), pool);
+    if(rc != MMC_OK)
+    {
+        mmc::releaseHeap(heap);
+        delete heap;
+        return rc;
+    }
+    *out = heap;
+    return MMC_OK;
+}
+
 int mmc_destroy(mmc_heap* heap)
 {
     if(!heap)
@@ -453,6 +490,8 @@ int mmc_destructive_resize(mmc_heap* heap, uint64_t bytes, void* stream)
 {
     if(!heap)
         return fail(MMC_EINVAL, "null heap");
+    if(!heap->ownsPool)
+        return fail(MMC_EINVAL, "the pool of this heap is caller-owned: destroy it and create a new one");
     (void) hipDeviceSynchronize();
     mmc::releaseHeap(heap);
     return mmc::initHeap(heap, bytes, static_cast<hipStream_t>(stream));

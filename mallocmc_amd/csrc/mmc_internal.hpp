@@ -13,7 +13,8 @@ struct mmc_heap
 {
     mmc_config cfg{};
     int device = 0;
-    void* rawPool = nullptr; // as returned by hipMalloc
+    void* rawPool = nullptr; // as returned by hipMalloc (nullptr for a caller-owned pool)
+    bool ownsPool = true;
     std::uint64_t rawBytes = 0;
     char* pool = nullptr; // after Shrink::alignPool
     std::uint64_t memsize = 0;

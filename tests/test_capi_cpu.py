@@ -74,3 +74,11 @@ def test_invalid_config_is_rejected(mmc):
     cfg = mmc.ScatterConfig(dataAlignment=24)
     with pytest.raises(mmc.MallocMCError):
         mmc.compute_geometry(cfg, 64 * MiB)
+
+
+def test_cpp_programs_built():
+    """The drop-in C++ headers compile into the example programs (built by build())."""
+    import os
+    d = os.path.join(REPO, "tests", "cpp", "_build")
+    for prog in ("example_arrays", "example_slots", "verify_heap"):
+        assert os.path.exists(os.path.join(d, prog)), prog

@@ -1,0 +1,38 @@
+"""GPU runs of the C++ programs written against the drop-in headers
+(include/mallocMC): the reference's example01 / example03 usage and its
+verify_heap harness (tests/verify_heap.cpp of the reference), built by
+__graft_entry__.build() into tests/cpp/_build/."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BUILD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build")
+
+
+def _run(prog, *args, timeout=120):
+    path = os.path.join(BUILD, prog)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is missing: run __graft_entry__.build()")
+    print(f"running {prog} {' '.join(args)}", flush=True)
+    r = subprocess.run([path, *args], capture_output=True, text=True, timeout=timeout)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    return r
+
+
+def test_example_arrays():
+    r = _run("example_arrays")
+    assert r.returncode == 0 and "example_arrays OK" in r.stdout
+
+
+def test_example_slots():
+    r = _run("example_slots")
+    assert r.returncode == 0 and "example_slots OK" in r.stdout
+
+
+@pytest.mark.parametrize("heap_mb,blocks,threads", [(1024, 64, 128), (256, 256, 256)])
+def test_verify_heap(heap_mb, blocks, threads):
+    r = _run("verify_heap", str(heap_mb), str(blocks), str(threads))
+    assert r.returncode == 0 and "verify_heap OK" in r.stdout
