@@ -29,6 +29,8 @@
  */
 #pragma once
 
+#include "device_allocator.hpp"
+#include "distribution.hpp"
 #include "scatter_heap.hpp"
 
 namespace mallocMC

@@ -40,6 +40,8 @@ EXPORTED_SYMBOLS = [
 BATCHED_SYMBOLS = [
     "mmc_default_tuning", "mmc_wave_malloc_batch", "mmc_wave_free_batch", "mmc_multi_create",
     "mmc_multi_destroy", "mmc_multi_step",
+    # include/mallocmc_serve.h, bound in mallocmc_amd/serve.py
+    "mmc_serve_create", "mmc_serve_destroy", "mmc_serve_load", "mmc_serve_step", "mmc_serve_get_stats",
 ]
 
 

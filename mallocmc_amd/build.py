@@ -21,6 +21,7 @@ SOURCES = [
     "mallocmc_capi.hip",
     "mallocmc_harness.hip",
     "mallocmc_batched.hip",
+    "mallocmc_serve.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -1,0 +1,368 @@
+// Device-resident serving loop: admission, decode (growing paged buffers on
+// the heap) and completion kernels.  Declarations: include/mallocmc_serve.h.
+#include "mallocmc_serve.h"
+
+#include "mallocMC/detail/wave_alloc.hpp"
+#include "mmc_internal.hpp"
+
+using namespace mallocMC;
+using namespace mallocMC::detail;
+
+namespace
+{
+    constexpr u32 kEmpty = 0xffffffffu;
+
+    enum Counter : u32
+    {
+        cAdmitted = 0,
+        cCompleted,
+        cMallocs,
+        cFrees,
+        cNulls,
+        cQueueWait,
+        cActive,
+        cMaxActive,
+        cCount
+    };
+
+    struct ServeScalars
+    {
+        u64 step;
+        u64 reserved;
+        u32 cursor;
+        u32 freeTop;
+        u32 nDone;
+        u32 pad;
+        u64 counters[cCount];
+    };
+
+    //! Everything a serving kernel needs, passed by value.
+    struct ServeView
+    {
+        ScatterHeap heap;
+        ServeScalars* sc;
+        u32* freeStack;
+        u32* slotReq;
+        u32* slotLen;
+        u32* slotCur;
+        u64* slotBlocks;
+        u32* doneList;
+        const u32* arrival;
+        const u32* length;
+        const u32* ids;
+        u64* checksums;
+        u32* admitStep;
+        u32 nRequests;
+        u32 maxSlots;
+        u32 blockBytes;
+        u32 tokensPerBlock;
+        u32 tokenBytes;
+        u32 maxBlocks;
+        u64 budget;
+    };
+
+    __device__ __forceinline__ u32 payloadWord(u32 r, u32 t, u32 w)
+    {
+        u32 x = r * 0x9E3779B1u ^ (t * 0x85EBCA77u) ^ (w * 0xC2B2AE3Du);
+        x ^= x >> 15;
+        x *= 0x2C1B3C6Du;
+        x ^= x >> 12;
+        return x;
+    }
+
+    //! Step part 1, one lane: admit arrived requests in order while a slot and
+    //! budget are available.
+    __global__ void admitKernel(ServeView v)
+    {
+        if(blockIdx.x != 0 || threadIdx.x != 0)
+            return;
+        ServeScalars* sc = v.sc;
+        sc->nDone = 0u;
+        const u64 step = sc->step;
+        u32 cursor = sc->cursor;
+        u64 reserved = sc->reserved;
+        u32 top = sc->freeTop;
+        while(cursor < v.nRequests && v.arrival[cursor] <= step)
+        {
+            const u32 len = v.length[cursor];
+            const u64 need = static_cast<u64>(divup(len, v.tokensPerBlock)) * v.blockBytes;
+            if(reserved + need > v.budget || top == 0u)
+                break;
+            const u32 slot = v.freeStack[--top];
+            v.slotReq[slot] = cursor;
+            v.slotLen[slot] = len;
+            v.slotCur[slot] = 0u;
+            reserved += need;
+            sc->counters[cAdmitted] += 1u;
+            sc->counters[cQueueWait] += step - v.arrival[cursor];
+            if(v.admitStep)
+                v.admitStep[cursor] = static_cast<u32>(step);
+            ++cursor;
+        }
+        sc->cursor = cursor;
+        sc->reserved = reserved;
+        sc->freeTop = top;
+        const u64 active = v.maxSlots - top;
+        sc->counters[cActive] = active;
+        if(active > sc->counters[cMaxActive])
+            sc->counters[cMaxActive] = active;
+        sc->step = step + 1u;
+    }
+
+    //! Step part 2, one lane per request slot: append one token.
+    __global__ __launch_bounds__(256) void decodeKernel(ServeView v)
+    {
+        const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
+        const bool valid = slot < v.maxSlots;
+        const u32 req = valid ? v.slotReq[slot] : kEmpty;
+        const bool active = req != kEmpty;
+        const u32 t = active ? v.slotCur[slot] : 0u;
+        const bool needBlock = active && (t % v.tokensPerBlock) == 0u;
+        const WaveAlloc wa{v.heap, WaveTuning{}};
+        void* p = wa.malloc<NoopDistribution>(needBlock ? v.blockBytes : 0u);
+        u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
+        u32 nulls = 0u;
+        if(needBlock)
+        {
+            blocks[t / v.tokensPerBlock] = v.heap.offsetOf(p);
+            if(p == nullptr)
+                nulls = 1u;
+        }
+        if(active)
+        {
+            const u64 off = blocks[t / v.tokensPerBlock];
+            if(off != kNullOffset)
+            {
+                const u32 id = v.ids[req];
+                u32* dst = reinterpret_cast<u32*>(v.heap.pages + off
+                                                  + static_cast<u64>(t % v.tokensPerBlock) * v.tokenBytes);
+                const u32 words = v.tokenBytes / 4u;
+                for(u32 w = 0; w < words; w += 4u)
+                {
+                    uint4 q;
+                    q.x = payloadWord(id, t, w);
+                    q.y = payloadWord(id, t, w + 1u);
+                    q.z = payloadWord(id, t, w + 2u);
+                    q.w = payloadWord(id, t, w + 3u);
+                    *reinterpret_cast<uint4*>(dst + w) = q;
+                }
+            }
+            const u32 t1 = t + 1u;
+            v.slotCur[slot] = t1;
+            if(t1 == v.slotLen[slot])
+                v.doneList[atom::add(&v.sc->nDone, 1u)] = slot;
+        }
+        u32 mallocs = needBlock ? 1u : 0u;
+        for(int d = 32; d > 0; d >>= 1)
+        {
+            mallocs += __shfl_xor(mallocs, d, 64);
+            nulls += __shfl_xor(nulls, d, 64);
+        }
+        if(hw::laneid() == 0u)
+        {
+            if(mallocs)
+                atom::add(&v.sc->counters[cMallocs], static_cast<u64>(mallocs));
+            if(nulls)
+                atom::add(&v.sc->counters[cNulls], static_cast<u64>(nulls));
+        }
+    }
+
+    //! Step part 3, one wave per finished request: checksum the paged buffer,
+    //! free its blocks, return slot and reservation.
+    __global__ __launch_bounds__(256) void completeKernel(ServeView v)
+    {
+        const u32 wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+        const u32 nWaves = (gridDim.x * blockDim.x) / 64u;
+        const u32 lane = hw::laneid();
+        const u32 nDone = atom::load(&v.sc->nDone);
+        const WaveAlloc wa{v.heap, WaveTuning{}};
+        for(u32 i = wave; i < nDone; i += nWaves)
+        {
+            const u32 slot = v.doneList[i];
+            const u32 req = v.slotReq[slot];
+            const u32 len = v.slotLen[slot];
+            const u32 nb = divup(len, v.tokensPerBlock);
+            const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
+            const u32 wpt = v.tokenBytes / 4u;
+            const u64 total = static_cast<u64>(len) * wpt;
+            u64 cs = 0u;
+            for(u64 k = lane; k < total; k += 64u)
+            {
+                const u32 t = static_cast<u32>(k / wpt);
+                const u32 w = static_cast<u32>(k % wpt);
+                const u64 off = blocks[t / v.tokensPerBlock];
+                if(off == kNullOffset)
+                    continue;
+                const u32 x = *reinterpret_cast<const u32*>(
+                    v.heap.pages + off + static_cast<u64>(t % v.tokensPerBlock) * v.tokenBytes + 4u * w);
+                cs += static_cast<u64>(x) * (2u * k + 1u);
+            }
+            for(int d = 32; d > 0; d >>= 1)
+                cs += __shfl_xor(cs, d, 64);
+            if(lane == 0u)
+                v.checksums[req] = cs;
+            u32 freed = 0u;
+            for(u32 b = 0; b < nb; b += 64u)
+            {
+                const u32 j = b + lane;
+                const u64 off = j < nb ? blocks[j] : kNullOffset;
+                wa.free(off == kNullOffset ? nullptr : v.heap.pages + off);
+                freed += off != kNullOffset ? 1u : 0u;
+            }
+            for(int d = 32; d > 0; d >>= 1)
+                freed += __shfl_xor(freed, d, 64);
+            if(lane == 0u)
+            {
+                atom::add(&v.sc->counters[cCompleted], u64{1});
+                atom::add(&v.sc->counters[cFrees], static_cast<u64>(freed));
+                atom::sub(&v.sc->reserved, static_cast<u64>(nb) * v.blockBytes);
+                v.slotReq[slot] = kEmpty;
+                const u32 pos = atom::add(&v.sc->freeTop, 1u);
+                v.freeStack[pos] = slot;
+            }
+        }
+    }
+
+    __global__ void resetKernel(ServeView v)
+    {
+        const u32 tid = blockIdx.x * blockDim.x + threadIdx.x;
+        const u32 stride = gridDim.x * blockDim.x;
+        for(u32 i = tid; i < v.maxSlots; i += stride)
+        {
+            v.freeStack[i] = v.maxSlots - 1u - i;
+            v.slotReq[i] = kEmpty;
+        }
+        if(tid == 0u)
+        {
+            v.sc->step = 0u;
+            v.sc->reserved = 0u;
+            v.sc->cursor = 0u;
+            v.sc->freeTop = v.maxSlots;
+            v.sc->nDone = 0u;
+            for(u32 c = 0; c < cCount; ++c)
+                v.sc->counters[c] = 0u;
+        }
+    }
+} // namespace
+
+struct mmc_serve
+{
+    mmc_heap* heap = nullptr;
+    mmc_serve_config cfg{};
+    ServeView view{};
+    void* arena = nullptr;
+};
+
+extern "C" {
+
+int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** out)
+{
+    if(!heap || !cfg || !out)
+        return mmc::fail(MMC_EINVAL, "null argument");
+    *out = nullptr;
+    if(cfg->max_slots == 0 || cfg->max_slots % 64u != 0 || cfg->block_bytes == 0 || cfg->tokens_per_block == 0
+       || cfg->token_bytes == 0 || cfg->token_bytes % 16u != 0 || cfg->max_tokens == 0
+       || static_cast<u64>(cfg->tokens_per_block) * cfg->token_bytes > cfg->block_bytes)
+        return mmc::fail(MMC_EINVAL, "bad serving configuration");
+    mmc_serve* s = new mmc_serve();
+    s->heap = heap;
+    s->cfg = *cfg;
+    const u32 maxBlocks = divup(cfg->max_tokens, cfg->tokens_per_block);
+    const u64 n = cfg->max_slots;
+    const u64 bytes = sizeof(ServeScalars) + 4u * n * 5u + 8u * n * maxBlocks + 256u;
+    hipError_t err = hipMalloc(&s->arena, bytes);
+    if(err != hipSuccess)
+    {
+        delete s;
+        return mmc::hipFail(err, "hipMalloc(serving state)");
+    }
+    char* a = static_cast<char*>(s->arena);
+    ServeView& v = s->view;
+    v.heap = heap->view;
+    v.sc = reinterpret_cast<ServeScalars*>(a);
+    a += (sizeof(ServeScalars) + 255u) & ~u64{255u};
+    v.slotBlocks = reinterpret_cast<u64*>(a);
+    a += 8u * n * maxBlocks;
+    v.freeStack = reinterpret_cast<u32*>(a);
+    a += 4u * n;
+    v.slotReq = reinterpret_cast<u32*>(a);
+    a += 4u * n;
+    v.slotLen = reinterpret_cast<u32*>(a);
+    a += 4u * n;
+    v.slotCur = reinterpret_cast<u32*>(a);
+    a += 4u * n;
+    v.doneList = reinterpret_cast<u32*>(a);
+    v.maxSlots = cfg->max_slots;
+    v.blockBytes = cfg->block_bytes;
+    v.tokensPerBlock = cfg->tokens_per_block;
+    v.tokenBytes = cfg->token_bytes;
+    v.maxBlocks = maxBlocks;
+    v.budget = cfg->budget_bytes;
+    *out = s;
+    return MMC_OK;
+}
+
+int mmc_serve_destroy(mmc_serve* s)
+{
+    if(!s)
+        return MMC_OK;
+    (void) hipDeviceSynchronize();
+    if(s->arena)
+        (void) hipFree(s->arena);
+    delete s;
+    return MMC_OK;
+}
+
+int mmc_serve_load(mmc_serve* s, const uint32_t* d_arrival, const uint32_t* d_length, const uint32_t* d_ids,
+                   uint32_t n, uint64_t* d_checksums, uint32_t* d_admit_step, void* stream)
+{
+    if(!s || (n && (!d_arrival || !d_length || !d_ids || !d_checksums)))
+        return mmc::fail(MMC_EINVAL, "null argument");
+    s->view.arrival = d_arrival;
+    s->view.length = d_length;
+    s->view.ids = d_ids;
+    s->view.checksums = d_checksums;
+    s->view.admitStep = d_admit_step;
+    s->view.nRequests = n;
+    hipLaunchKernelGGL(resetKernel, dim3(64), dim3(256), 0, static_cast<hipStream_t>(stream), s->view);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve reset launch");
+}
+
+int mmc_serve_step(mmc_serve* s, void* stream)
+{
+    if(!s)
+        return mmc::fail(MMC_EINVAL, "null argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(admitKernel, dim3(1), dim3(64), 0, st, s->view);
+    hipLaunchKernelGGL(decodeKernel, dim3(mmc::gridFor(s->cfg.max_slots, 256)), dim3(256), 0, st, s->view);
+    hipLaunchKernelGGL(completeKernel, dim3(256), dim3(256), 0, st, s->view);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve step launch");
+}
+
+int mmc_serve_get_stats(mmc_serve* s, mmc_serve_stats* out, void* stream)
+{
+    if(!s || !out)
+        return mmc::fail(MMC_EINVAL, "null argument");
+    ServeScalars h{};
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemcpyAsync(&h, s->view.sc, sizeof(h), hipMemcpyDeviceToHost, st);
+    if(err == hipSuccess)
+        err = hipStreamSynchronize(st);
+    if(err != hipSuccess)
+        return mmc::hipFail(err, "serve stats copy");
+    out->step = h.step;
+    out->admitted = h.counters[cAdmitted];
+    out->completed = h.counters[cCompleted];
+    out->block_mallocs = h.counters[cMallocs];
+    out->block_frees = h.counters[cFrees];
+    out->nulls = h.counters[cNulls];
+    out->queue_wait_steps = h.counters[cQueueWait];
+    out->active = s->cfg.max_slots - h.freeTop;
+    out->reserved_bytes = h.reserved;
+    out->max_active = h.counters[cMaxActive];
+    return MMC_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,225 @@
+"""Serving loop driver (config 5): Poisson arrivals, growing paged buffers,
+admission by free heap bytes, p50/p99 step latency.
+
+The loop itself runs on the device (include/mallocmc_serve.h); this module
+builds the seeded request trace, captures one step in a hipGraph, replays it
+for every simulated step with HIP events around each replay, and checks
+request checksums against their closed form.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+import mallocmc_amd as mmc
+
+GiB = 1024 ** 3
+MiB = 1024 ** 2
+M32 = np.uint64(0xFFFFFFFF)
+
+
+class MmcServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("max_slots", ctypes.c_uint32),
+        ("block_bytes", ctypes.c_uint32),
+        ("tokens_per_block", ctypes.c_uint32),
+        ("token_bytes", ctypes.c_uint32),
+        ("max_tokens", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("budget_bytes", ctypes.c_uint64),
+    ]
+
+
+class MmcServeStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "step", "admitted", "completed", "block_mallocs", "block_frees", "nulls", "queue_wait_steps", "active",
+        "reserved_bytes", "max_active")]
+
+
+SERVE_SYMBOLS = ["mmc_serve_create", "mmc_serve_destroy", "mmc_serve_load", "mmc_serve_step", "mmc_serve_get_stats"]
+
+
+def _bind():
+    L = mmc.lib()
+    if getattr(L, "_serve_bound", False):
+        return L
+    P = ctypes.POINTER
+    vp, u32 = ctypes.c_void_p, ctypes.c_uint32
+    L.mmc_serve_create.argtypes = [vp, P(MmcServeConfig), P(vp)]
+    L.mmc_serve_destroy.argtypes = [vp]
+    L.mmc_serve_load.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp]
+    L.mmc_serve_step.argtypes = [vp, vp]
+    L.mmc_serve_get_stats.argtypes = [vp, P(MmcServeStats), vp]
+    for n in SERVE_SYMBOLS:
+        getattr(L, n).restype = ctypes.c_int
+    L._serve_bound = True
+    return L
+
+
+@dataclasses.dataclass
+class Trace:
+    arrival: np.ndarray  # uint32, sorted
+    length: np.ndarray   # uint32 tokens
+    ids: np.ndarray      # uint32 payload ids
+
+
+def make_trace(seed: int = 5, rate_per_s: float = 2000.0, step_s: float = 0.01, duration_s: float = 30.0,
+               mu: float = 6.2, sigma: float = 0.8, max_tokens: int = 8192) -> Trace:
+    """Poisson arrivals (rate_per_s) over duration_s in steps of step_s; lengths
+    lognormal(mu, sigma) tokens clipped to [1, max_tokens]."""
+    rng = np.random.default_rng(seed)
+    steps = int(round(duration_s / step_s))
+    per_step = rng.poisson(rate_per_s * step_s, size=steps)
+    arrival = np.repeat(np.arange(steps, dtype=np.uint32), per_step)
+    length = np.clip(np.rint(rng.lognormal(mu, sigma, size=len(arrival))), 1, max_tokens).astype(np.uint32)
+    return Trace(arrival, length, np.arange(len(arrival), dtype=np.uint32))
+
+
+def payload_words(rid: int, t: np.ndarray, w: np.ndarray) -> np.ndarray:
+    """mmc_serve_word (include/mallocmc_serve.h) over arrays, uint32 arithmetic."""
+    with np.errstate(over="ignore"):
+        x = (np.uint64(rid) * np.uint64(0x9E3779B1)) & M32
+        x = x ^ ((t.astype(np.uint64) * np.uint64(0x85EBCA77)) & M32) ^ ((w.astype(np.uint64) * np.uint64(0xC2B2AE3D)) & M32)
+        x = x ^ (x >> np.uint64(15))
+        x = (x * np.uint64(0x2C1B3C6D)) & M32
+        x = x ^ (x >> np.uint64(12))
+    return x
+
+
+def expected_checksum(rid: int, length: int, token_bytes: int = 256) -> int:
+    wpt = token_bytes // 4
+    k = np.arange(length * wpt, dtype=np.uint64)
+    x = payload_words(rid, k // np.uint64(wpt), k % np.uint64(wpt))
+    with np.errstate(over="ignore"):
+        return int(np.sum(x * (np.uint64(2) * k + np.uint64(1)), dtype=np.uint64))
+
+
+class ServeLoop:
+    def __init__(self, allocator: mmc.ScatterAllocator, max_slots: int = 32768, block_bytes: int = 4096,
+                 tokens_per_block: int = 16, token_bytes: int = 256, max_tokens: int = 8192,
+                 budget_bytes: int | None = None):
+        L = _bind()
+        self.alloc = allocator
+        g = allocator.geometry()
+        if budget_bytes is None:
+            budget_bytes = int(g.numpages) * allocator.config.pagesize * 9 // 10
+        self.cfg = MmcServeConfig(max_slots, block_bytes, tokens_per_block, token_bytes, max_tokens, 0, budget_bytes)
+        self.token_bytes = token_bytes
+        self._s = ctypes.c_void_p()
+        mmc._check(L.mmc_serve_create(allocator.handle, ctypes.byref(self.cfg), ctypes.byref(self._s)),
+                   "mmc_serve_create")
+        self._keep = []
+
+    def close(self):
+        if self._s:
+            _bind().mmc_serve_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, trace: Trace, stream=None):
+        import torch
+        dev = "cuda"
+        self.trace = trace
+        self.d_arrival = torch.from_numpy(trace.arrival.astype(np.int32)).to(dev)
+        self.d_length = torch.from_numpy(trace.length.astype(np.int32)).to(dev)
+        self.d_ids = torch.from_numpy(trace.ids.astype(np.int32)).to(dev)
+        n = len(trace.arrival)
+        self.d_checksums = torch.zeros(max(n, 1), dtype=torch.int64, device=dev)
+        self.d_admit = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        mmc._check(_bind().mmc_serve_load(self._s, ctypes.c_void_p(self.d_arrival.data_ptr()),
+                                          ctypes.c_void_p(self.d_length.data_ptr()),
+                                          ctypes.c_void_p(self.d_ids.data_ptr()), n,
+                                          ctypes.c_void_p(self.d_checksums.data_ptr()),
+                                          ctypes.c_void_p(self.d_admit.data_ptr()), mmc._stream_ptr(stream)),
+                   "mmc_serve_load")
+
+    def step(self, stream=None):
+        mmc._check(_bind().mmc_serve_step(self._s, mmc._stream_ptr(stream)), "mmc_serve_step")
+
+    def stats(self, stream=None) -> dict:
+        st = MmcServeStats()
+        mmc._check(_bind().mmc_serve_get_stats(self._s, ctypes.byref(st), mmc._stream_ptr(stream)),
+                   "mmc_serve_get_stats")
+        return {n: getattr(st, n) for n, _ in MmcServeStats._fields_}
+
+    def checksums(self) -> np.ndarray:
+        return self.d_checksums.cpu().numpy().view(np.uint64)[: len(self.trace.arrival)]
+
+    def run(self, steps: int, graph: bool = True, time_steps: bool = True):
+        """Runs `steps` steps; returns per-step GPU milliseconds (or None)."""
+        import torch
+        if graph:
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    self.step(stream=s)
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+            launch = g.replay
+        else:
+            launch = self.step
+        events = None
+        if time_steps:
+            events = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+            events[0].record()
+        for i in range(steps):
+            launch()
+            if time_steps:
+                events[i + 1].record()
+        torch.cuda.synchronize()
+        if not time_steps:
+            return None
+        return np.array([events[i].elapsed_time(events[i + 1]) for i in range(steps)])
+
+
+def run_default(log=None, duration_s: float = 30.0, check_sample: int = 64) -> dict:
+    """Config 5: 32 GiB heap, 2000 req/s Poisson over 30 s in 10 ms steps,
+    lognormal(6.2, 0.8) tokens (<= 8192) at 256 B/token, 4 KiB blocks of 16
+    tokens, admission by free bytes.  Runs until every request completed."""
+    import torch
+    cfg = mmc.ScatterConfig(pagesize=16384, accessblocksize=2 * GiB, regionsize=16, resetfreedpages=True,
+                            flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
+    alloc = mmc.ScatterAllocator(cfg, 32 * GiB)
+    slots0 = alloc.getAvailableSlots(4096)
+    loop = ServeLoop(alloc, max_slots=32768)
+    trace = make_trace(duration_s=duration_s)
+    loop.load(trace)
+    arrival_steps = int(trace.arrival.max()) + 1
+    tail = int(trace.length.max()) + 2
+    ms = loop.run(arrival_steps + tail)
+    st = loop.stats()
+    cs = loop.checksums()
+    rng = np.random.default_rng(1)
+    sample = rng.choice(len(trace.arrival), size=min(check_sample, len(trace.arrival)), replace=False)
+    bad = sum(int(cs[i]) != expected_checksum(int(trace.ids[i]), int(trace.length[i])) for i in sample)
+    torch.cuda.synchronize()
+    slots_ok = alloc.getAvailableSlots(4096) == slots0
+    loop.close()
+    alloc.close()
+    busy = ms[:arrival_steps]  # steps while arrivals continue (steady state)
+    total_s = float(ms.sum()) / 1e3
+    out = {
+        "workload": f"config 5: Poisson 2000 req/s for {duration_s:.0f} s in 10 ms steps, lognormal(6.2,0.8) tokens "
+                    "<= 8192, 256 B/token, 4 KiB blocks, 32 GiB heap, pagesize 16 KiB",
+        "requests": int(len(trace.arrival)), "steps": int(len(ms)), "arrival_steps": arrival_steps,
+        "p50_step_us": float(np.percentile(busy, 50) * 1e3), "p99_step_us": float(np.percentile(busy, 99) * 1e3),
+        "max_step_us": float(busy.max() * 1e3),
+        "steps_per_s": len(ms) / total_s, "allocs_per_s": st["block_mallocs"] / total_s,
+        "frees_per_s": st["block_frees"] / total_s,
+        "completed": int(st["completed"]), "admitted": int(st["admitted"]), "nulls": int(st["nulls"]),
+        "max_active": int(st["max_active"]), "mean_queue_wait_steps": st["queue_wait_steps"] / max(1, st["admitted"]),
+        "checksum_mismatches_in_sample": int(bad), "checksum_sample": int(len(sample)),
+        "slots_restored": bool(slots_ok),
+    }
+    if log:
+        log(f"serve: {out}")
+    return out

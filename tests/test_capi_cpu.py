@@ -82,3 +82,46 @@ def test_cpp_programs_built():
     d = os.path.join(REPO, "tests", "cpp", "_build")
     for prog in ("example_arrays", "example_slots", "verify_heap"):
         assert os.path.exists(os.path.join(d, prog)), prog
+
+
+def test_serve_payload_word_matches_header(tmp_path):
+    """The host closed form of the serving payload (mallocmc_amd/serve.py)
+    equals mmc_serve_word of include/mallocmc_serve.h compiled with gcc."""
+    import subprocess
+    import numpy as np
+    from mallocmc_amd import serve
+    src = tmp_path / "w.c"
+    src.write_text('#include "mallocmc_serve.h"\nunsigned w(unsigned r, unsigned t, unsigned x)'
+                   '{ return mmc_serve_word(r, t, x); }\n')
+    so = tmp_path / "w.so"
+    subprocess.run(["gcc", "-O2", "-shared", "-fPIC", "-I", os.path.join(REPO, "include"), str(src), "-o",
+                    str(so)], check=True)
+    L = ctypes.CDLL(str(so))
+    L.w.restype = ctypes.c_uint32
+    L.w.argtypes = [ctypes.c_uint32] * 3
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        r, t, x = (int(v) for v in rng.integers(0, 2 ** 32, size=3, dtype=np.uint64))
+        got = int(serve.payload_words(r, np.array([t], np.uint64), np.array([x], np.uint64))[0])
+        assert got == L.w(r, t, x)
+
+
+def test_serve_checksum_closed_form_small():
+    """expected_checksum against a direct loop for tiny requests."""
+    import numpy as np
+    from mallocmc_amd import serve
+    for rid, length in ((0, 1), (7, 3), (123456, 5)):
+        acc = 0
+        for k in range(length * 64):
+            x = int(serve.payload_words(rid, np.array([k // 64], np.uint64), np.array([k % 64], np.uint64))[0])
+            acc = (acc + x * (2 * k + 1)) % (1 << 64)
+        assert serve.expected_checksum(rid, length) == acc
+
+
+def test_serve_trace_shape():
+    import numpy as np
+    from mallocmc_amd import serve
+    t = serve.make_trace(seed=5)
+    assert (np.diff(t.arrival.astype(np.int64)) >= 0).all()
+    assert t.length.min() >= 1 and t.length.max() <= 8192
+    assert abs(len(t.arrival) / 30.0 - 2000) < 100  # Poisson 2000/s over 30 s

@@ -1,0 +1,70 @@
+"""GPU tests of the device-resident serving loop (include/mallocmc_serve.h).
+
+Every completed request's checksum is read back from its paged buffer on the
+heap and must equal the closed form; the same requests run alone must give
+the same checksums as in the mix (exact: integer payloads); admission keeps
+reserved bytes within the budget and the heap returns to its initial slot
+count once every request completed.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MiB = 1024 ** 2
+
+
+def _alloc(mmc, heap=256 * MiB):
+    cfg = mmc.ScatterConfig(pagesize=16384, accessblocksize=64 * MiB, regionsize=16, resetfreedpages=True,
+                            flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
+    return mmc.ScatterAllocator(cfg, heap)
+
+
+def _run_all(loop, trace):
+    loop.load(trace)
+    steps = int(trace.arrival.max()) + int(trace.length.max()) + 2 if len(trace.arrival) else 1
+    loop.run(steps, graph=True, time_steps=False)
+    return loop.stats()
+
+
+def test_serve_checksums_and_restore(mmc, cuda):
+    from mallocmc_amd import serve
+    a = _alloc(mmc)
+    s0 = a.getAvailableSlots(4096)
+    loop = serve.ServeLoop(a, max_slots=4096, max_tokens=2048)
+    trace = serve.make_trace(seed=3, rate_per_s=2000, step_s=0.01, duration_s=1.0, mu=5.0, sigma=0.8,
+                             max_tokens=2048)
+    print(f"serve trace: {len(trace.arrival)} requests", flush=True)
+    st = _run_all(loop, trace)
+    assert st["completed"] == len(trace.arrival) and st["nulls"] == 0, st
+    assert st["block_mallocs"] == st["block_frees"] == int(np.sum((trace.length + 15) // 16))
+    cs = loop.checksums()
+    for i in range(len(trace.arrival)):
+        assert int(cs[i]) == serve.expected_checksum(int(trace.ids[i]), int(trace.length[i])), i
+    cuda.cuda.synchronize()
+    assert a.getAvailableSlots(4096) == s0
+    loop.close()
+    a.close()
+
+
+def test_serve_alone_equals_mix_under_admission_pressure(mmc, cuda):
+    """A budget of 8 MiB forces requests to wait for bytes to free up; the
+    checksums of requests run alone equal those in the mix."""
+    from mallocmc_amd import serve
+    a = _alloc(mmc)
+    loop = serve.ServeLoop(a, max_slots=1024, max_tokens=1024, budget_bytes=8 * MiB)
+    trace = serve.make_trace(seed=4, rate_per_s=4000, step_s=0.01, duration_s=0.5, mu=5.5, sigma=0.6,
+                             max_tokens=1024)
+    st = _run_all(loop, trace)
+    assert st["completed"] == len(trace.arrival) and st["nulls"] == 0, st
+    assert st["queue_wait_steps"] > 0, "the budget must make requests wait"
+    mix = loop.checksums().copy()
+    admit = loop.d_admit.cpu().numpy()[: len(trace.arrival)]
+    assert (admit >= trace.arrival).all()
+    for i in (0, len(trace.arrival) // 2, len(trace.arrival) - 1):
+        alone = serve.Trace(trace.arrival[i:i + 1] * 0, trace.length[i:i + 1], trace.ids[i:i + 1])
+        _run_all(loop, alone)
+        assert int(loop.checksums()[0]) == int(mix[i]) == serve.expected_checksum(int(trace.ids[i]),
+                                                                                   int(trace.length[i]))
+    loop.close()
+    a.close()
