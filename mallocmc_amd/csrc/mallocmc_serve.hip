@@ -92,6 +92,7 @@ namespace
             v.slotReq[slot] = cursor;
             v.slotLen[slot] = len;
             v.slotCur[slot] = 0u;
+            v.checksums[cursor] = 0u;
             reserved += need;
             sc->counters[cAdmitted] += 1u;
             sc->counters[cQueueWait] += step - v.arrival[cursor];
@@ -167,9 +168,84 @@ namespace
         }
     }
 
-    //! Step part 3, one wave per finished request: checksum the paged buffer,
-    //! free its blocks, return slot and reservation.
-    __global__ __launch_bounds__(256) void completeKernel(ServeView v)
+    constexpr u32 kTokensPerChunk = 64u;
+
+    //! Step part 3a: the paged buffers of the finished requests, cut into
+    //! 64-token chunks spread over the whole grid, are read back into their
+    //! checksums (lane = word of a token, the 4 waves of a workgroup take every
+    //! fourth token, 4 tokens in flight per lane; partial sums add atomically).
+    __global__ __launch_bounds__(256) void checksumKernel(ServeView v)
+    {
+        __shared__ u64 partial[4];
+        const u32 lane = hw::laneid();
+        const u32 q = threadIdx.x / 64u;
+        const u32 nDone = atom::load(&v.sc->nDone);
+        const u32 wpt = v.tokenBytes / 4u;
+        u32 i = 0u, base = 0u; // done-list entry and its first chunk id
+        for(u32 c = blockIdx.x;; c += gridDim.x)
+        {
+            u32 nChunks = 0u;
+            while(i < nDone)
+            {
+                nChunks = divup(v.slotLen[v.doneList[i]], kTokensPerChunk);
+                if(c < base + nChunks)
+                    break;
+                base += nChunks;
+                ++i;
+            }
+            if(i >= nDone)
+                break;
+            const u32 slot = v.doneList[i];
+            const u32 req = v.slotReq[slot];
+            const u32 len = v.slotLen[slot];
+            const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
+            const u32 t0 = (c - base) * kTokensPerChunk;
+            const u32 t1 = umin32(len, t0 + kTokensPerChunk);
+            u64 cs = 0u;
+            for(u32 w = lane; w < wpt; w += 64u)
+            {
+                u32 t = t0 + q;
+                for(; t + 12u < t1; t += 16u)
+                {
+                    u32 x[4];
+#pragma unroll
+                    for(u32 j = 0; j < 4u; ++j)
+                    {
+                        const u32 tt = t + 4u * j;
+                        const u64 off = blocks[tt / v.tokensPerBlock];
+                        x[j] = off == kNullOffset
+                            ? 0u
+                            : *reinterpret_cast<const u32*>(
+                                v.heap.pages + off + static_cast<u64>(tt % v.tokensPerBlock) * v.tokenBytes + 4u * w);
+                    }
+#pragma unroll
+                    for(u32 j = 0; j < 4u; ++j)
+                        cs += static_cast<u64>(x[j]) * (2u * (static_cast<u64>(t + 4u * j) * wpt + w) + 1u);
+                }
+                for(; t < t1; t += 4u)
+                {
+                    const u64 off = blocks[t / v.tokensPerBlock];
+                    if(off == kNullOffset)
+                        continue;
+                    const u32 x = *reinterpret_cast<const u32*>(
+                        v.heap.pages + off + static_cast<u64>(t % v.tokensPerBlock) * v.tokenBytes + 4u * w);
+                    cs += static_cast<u64>(x) * (2u * (static_cast<u64>(t) * wpt + w) + 1u);
+                }
+            }
+            for(int d = 32; d > 0; d >>= 1)
+                cs += __shfl_xor(cs, d, 64);
+            if(lane == 0u)
+                partial[q] = cs;
+            __syncthreads();
+            if(threadIdx.x == 0u)
+                atom::add(&v.checksums[req], partial[0] + partial[1] + partial[2] + partial[3]);
+            __syncthreads();
+        }
+    }
+
+    //! Step part 3b, one wave per finished request: free its blocks, return
+    //! slot and reservation.
+    __global__ __launch_bounds__(256) void releaseKernel(ServeView v)
     {
         const u32 wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
         const u32 nWaves = (gridDim.x * blockDim.x) / 64u;
@@ -179,28 +255,8 @@ namespace
         for(u32 i = wave; i < nDone; i += nWaves)
         {
             const u32 slot = v.doneList[i];
-            const u32 req = v.slotReq[slot];
-            const u32 len = v.slotLen[slot];
-            const u32 nb = divup(len, v.tokensPerBlock);
+            const u32 nb = divup(v.slotLen[slot], v.tokensPerBlock);
             const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
-            const u32 wpt = v.tokenBytes / 4u;
-            const u64 total = static_cast<u64>(len) * wpt;
-            u64 cs = 0u;
-            for(u64 k = lane; k < total; k += 64u)
-            {
-                const u32 t = static_cast<u32>(k / wpt);
-                const u32 w = static_cast<u32>(k % wpt);
-                const u64 off = blocks[t / v.tokensPerBlock];
-                if(off == kNullOffset)
-                    continue;
-                const u32 x = *reinterpret_cast<const u32*>(
-                    v.heap.pages + off + static_cast<u64>(t % v.tokensPerBlock) * v.tokenBytes + 4u * w);
-                cs += static_cast<u64>(x) * (2u * k + 1u);
-            }
-            for(int d = 32; d > 0; d >>= 1)
-                cs += __shfl_xor(cs, d, 64);
-            if(lane == 0u)
-                v.checksums[req] = cs;
             u32 freed = 0u;
             for(u32 b = 0; b < nb; b += 64u)
             {
@@ -336,7 +392,8 @@ int mmc_serve_step(mmc_serve* s, void* stream)
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(admitKernel, dim3(1), dim3(64), 0, st, s->view);
     hipLaunchKernelGGL(decodeKernel, dim3(mmc::gridFor(s->cfg.max_slots, 256)), dim3(256), 0, st, s->view);
-    hipLaunchKernelGGL(completeKernel, dim3(256), dim3(256), 0, st, s->view);
+    hipLaunchKernelGGL(checksumKernel, dim3(1024), dim3(256), 0, st, s->view);
+    hipLaunchKernelGGL(releaseKernel, dim3(128), dim3(256), 0, st, s->view);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve step launch");
 }

@@ -20,10 +20,16 @@ def _alloc(mmc, heap=256 * MiB):
     return mmc.ScatterAllocator(cfg, heap)
 
 
-def _run_all(loop, trace):
+def _run_all(loop, trace, chunk=512, max_steps=200000):
+    """Steps until every request completed (admission may make them wait)."""
     loop.load(trace)
-    steps = int(trace.arrival.max()) + int(trace.length.max()) + 2 if len(trace.arrival) else 1
-    loop.run(steps, graph=True, time_steps=False)
+    done = 0
+    while done < max_steps:
+        loop.run(chunk, graph=True, time_steps=False)
+        done += chunk
+        st = loop.stats()
+        if st["completed"] == len(trace.arrival):
+            return st
     return loop.stats()
 
 
