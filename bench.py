@@ -16,8 +16,9 @@ Forms:
 
 Also reported: the device-atomic ceiling (microbenchmark), the roofline of
 the dominant kernel, the CPU baseline (the oracle on this host's cores, the
-reference's CpuOmp2Blocks execution model with 16-way mutex atomics) and the
-serving-loop p50/p99 step latency.
+reference's CpuOmp2Blocks execution model with 16-way mutex atomics), the
+serving-loop p50/p99 step latency (config 5, mallocmc_amd/serve.py) and the
+multi-heap batched form (config 3, mallocmc_amd/multiheap.py).
 """
 from __future__ import annotations
 
@@ -128,6 +129,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="use the stored best point instead of tuning")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-multi", action="store_true", help="skip the multi-heap (config 3) section")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -251,6 +253,11 @@ def main():
         plain = measure_plain(torch, mmc, batches, args.plain_steps)
         log(f"plain: {plain}")
 
+    multi = None
+    if not args.no_multi:
+        from mallocmc_amd import multiheap
+        multi = multiheap.run_config3(log=log, tune_steps=10)
+
     serve = None
     if not args.no_serve:
         try:
@@ -303,6 +310,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "serve": serve,
+        "batched_multi_heap": multi,
     }
     print(json.dumps(out), flush=True)
 

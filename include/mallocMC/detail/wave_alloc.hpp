@@ -283,14 +283,18 @@ namespace mallocMC
                 return served;
             }
 
-            //! Group allocation of chunk size c for the lanes in `group`.
+            //! Group allocation of chunk size c for the lanes in `group`.  Walks
+            //! the regions in the reference's order (Scatter.hpp:563-635: from the
+            //! hashed page, region by region, two passes with the 3/4 fill
+            //! threshold first) but examines up to 64 pages -- 64/regionsize
+            //! regions -- per round trip.
             MALLOCMC_D u64 allocGroup(u64 group, u32 c, u32 bmax, u32 maxcs, char*& out) const
             {
                 const u32 me = hw::laneid();
                 const u32 R = h.regionsize;
                 const u32 ppb = h.pagesperblock;
-                const u32 numpages = h.numpages;
-                u32 hashBase = c * h.hashingK + h.hashingDistMP * hw::cuid()
+                const u32 numregions = h.numpages / R;
+                const u32 hashBase = c * h.hashingK + h.hashingDistMP * hw::cuid()
                     + (h.hashingDistWP + h.hashingDistWPRel * (64u * c / h.pagesize)) * hw::waveslot();
                 u32 startInBlock;
                 if(t.xcdSlices > 1u && ppb >= t.xcdSlices * R)
@@ -305,48 +309,63 @@ namespace mallocMC
                     ffb = atom::load(&h.scalars->firstfreeblock);
                 ffb = wave::lane(ffb, 0u);
                 const u32 globalStart = startInBlock + ffb * ppb;
-
+                const u32 region0 = globalStart / R;
+                const u32 off0 = globalStart % R; // every region is entered at this page
+                // regions examined per round trip
+                const u32 W = R >= 64u ? 1u : 64u / R;
+                const u32 span = R >= 64u ? 64u : W * R;
                 u64 need = group;
                 u32 checklevel = R * 3u / 4u;
                 for(u32 finder = 0; finder < 2u && need != 0u; ++finder)
                 {
-                    u32 gp = globalStart;
-                    do
+                    for(u32 done = 0; done < numregions && need != 0u; done += W)
                     {
-                        const u32 region = gp / R;
-                        const u32 regionOffset = region * R;
-                        u32 fill = 0u;
-                        if(me == 0u)
-                            fill = atom::load(&h.regions[region]);
-                        fill = wave::lane(fill, 0u);
-                        if(fill < checklevel)
+                        const u32 nreg = umin32(W, numregions - done);
+                        // lane -> (region of this round, page within region)
+                        const u32 ri = R >= 64u ? 0u : me / R;
+                        const u32 region = (region0 + done + ri) % numregions;
+                        u32 fill = 0xffffffffu, pcs = 0u, pcount = 0u;
+                        const bool inSpan = me < span && ri < nreg;
+                        for(u32 base = 0; base < (R >= 64u ? R : 1u) && need != 0u; base += 64u)
                         {
-                            const u64 before = need;
-                            for(u32 base = 0; base < R && need != 0u; base += 64u)
+                            const u32 pj = R >= 64u ? base + me : me % R;
+                            const bool valid = inSpan && pj < R;
+                            if(valid)
                             {
-                                const u32 j = base + me;
-                                u32 pcs = 0u, pcount = 0u;
-                                const bool valid = j < R;
-                                if(valid)
-                                {
-                                    pcs = atom::load(h.pteChunksize(regionOffset + j));
-                                    pcount = atom::load(h.pteCount(regionOffset + j));
-                                }
-                                bool cand = false;
-                                if(valid)
-                                {
-                                    if(pcs == 0u)
-                                        cand = true;
-                                    else if(pcs >= bmax && pcs <= maxcs && pcs <= h.pagesize)
-                                        cand = pcount < h.pageCapacity(pcs);
-                                }
+                                fill = atom::load(&h.regions[region]);
+                                pcs = atom::load(h.pteChunksize(region * R + pj));
+                                pcount = atom::load(h.pteCount(region * R + pj));
+                            }
+                            const bool regionOpen = valid && fill < checklevel;
+                            bool cand = false;
+                            if(regionOpen)
+                            {
+                                if(pcs == 0u)
+                                    cand = true;
+                                else if(pcs >= bmax && pcs <= maxcs && pcs <= h.pagesize)
+                                    cand = pcount < h.pageCapacity(pcs);
+                            }
+                            const u64 usedAll = hw::ballot(cand && pcs != 0u);
+                            const u64 freeAll = hw::ballot(cand && pcs == 0u);
+                            const u64 open = hw::ballot(regionOpen);
+                            const u32 rounds = R >= 64u ? 1u : nreg;
+                            for(u32 k = 0; k < rounds && need != 0u; ++k)
+                            {
+                                const u32 lo = R >= 64u ? 0u : k * R;
+                                const u32 width = R >= 64u ? umin32(64u, R - base) : R;
+                                const u64 sel = (width >= 64u ? ~0ull : ((1ull << width) - 1u)) << lo;
+                                if((open & sel) == 0u)
+                                    continue; // region filled past the threshold
+                                const u32 fillK = wave::lane(fill, lo);
+                                const u32 regionK = R >= 64u ? region : (region0 + done + k) % numregions;
+                                u64 used = usedAll & sel;
+                                u64 cands = freeAll & sel;
+                                const u32 rotAbs = R >= 64u ? (off0 >= base ? off0 - base : 0u) : off0;
+                                const u32 rot = lo + rotAbs;
+                                const u64 before = need;
                                 // partially used pages of a fitting chunk size first, free
                                 // pages after them: formatting a fresh page while a fitting
                                 // one has room is what fragments the reference's heap
-                                u64 used = hw::ballot(cand && pcs != 0u);
-                                u64 cands = hw::ballot(cand && pcs == 0u);
-                                // cyclic order from the hashed page within the region
-                                const u32 rot = (gp - regionOffset) >= base ? (gp - regionOffset) - base : 0u;
                                 while((used | cands) != 0u && need != 0u)
                                 {
                                     const u64 set = used != 0u ? used : cands;
@@ -355,7 +374,7 @@ namespace mallocMC
                                     const u32 l = hw::ffs(pick) - 1u;
                                     used &= ~(1ull << l);
                                     cands &= ~(1ull << l);
-                                    const u32 page = regionOffset + base + l;
+                                    const u32 page = regionK * R + (R >= 64u ? base + l : l - lo);
                                     u32 cs = wave::lane(pcs, l);
                                     bool formatted = false;
                                     if(cs == 0u)
@@ -377,21 +396,93 @@ namespace mallocMC
                                     const u64 served = claimOnPage(page, cs, bmax, maxcs, need, formatted, out);
                                     need &= ~served;
                                 }
+                                if(need == before && fillK + 1u <= R && me == 0u)
+                                    atom::cas(&h.regions[regionK], fillK, fillK + 1u);
                             }
-                            if(need == before && fill + 1u <= R && me == 0u)
-                                atom::cas(&h.regions[region], fill, fill + 1u);
                         }
-                        gp = (gp + R) % numpages;
-                        if(gp % ppb == 0u && me == 0u)
+                        // access-block cursor: advanced when the walk enters a new block
+                        // (Scatter.hpp:620-629); lane i looks at the region after region i
+                        // of this round
+                        u32 abid = 0u;
+                        if(me < nreg)
                         {
-                            const u32 abid = gp / ppb;
-                            if(abid > atom::load(&h.scalars->firstfreeblock))
-                                atom::store(&h.scalars->firstfreeblock, abid);
+                            const u32 rg = (region0 + done + me + 1u) % numregions;
+                            if((rg * R) % ppb == 0u)
+                                abid = rg * R / ppb;
                         }
-                    } while(gp != globalStart && need != 0u);
+                        abid = wave::max(abid);
+                        if(abid != 0u && me == 0u && abid > atom::load(&h.scalars->firstfreeblock))
+                            atom::store(&h.scalars->firstfreeblock, abid);
+                    }
                     checklevel = R + 1u;
                 }
                 return group & ~need;
+            }
+
+            //! Multi-page request of `bytes` > pagesize for one lane, served by the
+            //! whole wave (uniform `bytes`).  The reference takes a global spin
+            //! mutex and scans one PTE at a time from _firstFreePageBased down
+            //! (Scatter.hpp:775-840); here the wave reads 64 PTEs per round trip
+            //! in the same top-down order, finds runs of free pages with ballot
+            //! arithmetic and claims one with markpages, whose CAS-and-roll-back
+            //! (Scatter.hpp:745-764) is safe without the mutex.
+            MALLOCMC_D char* allocRun(u32 bytes) const
+            {
+                const u32 me = hw::laneid();
+                const u32 P = h.pagesize;
+                const u32 numpages = h.numpages;
+                const u32 k = divup(bytes, P);
+                if(k > 32u)
+                {
+                    // very long runs: the reference's serialized path on one lane
+                    u64 r = 0u;
+                    if(me == 0u)
+                        r = reinterpret_cast<u64>(h.allocPageBasedSingle(bytes));
+                    return reinterpret_cast<char*>(hw::readlane64(r, 0u));
+                }
+                u32 start = 0u;
+                if(me == 0u)
+                    start = atom::load(&h.scalars->firstFreePageBased);
+                start = wave::lane(start, 0u);
+                if(start >= numpages)
+                    start = numpages - 1u;
+                for(u32 sweep = 0; sweep < 2u; ++sweep)
+                {
+                    long long top = sweep == 0u ? static_cast<long long>(start) : static_cast<long long>(numpages) - 1;
+                    const long long bottom = sweep == 0u ? 0 : static_cast<long long>(start);
+                    while(top >= bottom)
+                    {
+                        const long long base = top - 63 > bottom ? top - 63 : bottom;
+                        const long long pg = base + me;
+                        bool isFree = false;
+                        if(pg <= top)
+                            isFree = atom::load(h.pteChunksize(static_cast<u32>(pg))) == 0u;
+                        const u64 m = hw::ballot(isFree);
+                        u64 r = m;
+                        for(u32 j = 1; j < k; ++j)
+                            r &= m >> j;
+                        while(r != 0u)
+                        {
+                            const u32 pos = 63u - static_cast<u32>(__builtin_clzll(r));
+                            const u32 sp = static_cast<u32>(base) + pos;
+                            u32 ok = 0u;
+                            if(me == 0u)
+                                ok = h.markpages(sp, k, bytes) ? 1u : 0u;
+                            ok = wave::lane(ok, 0u);
+                            if(ok)
+                            {
+                                if(me == 0u && sp + k - 1u == start && sp > 0u)
+                                    atom::cas(&h.scalars->firstFreePageBased, start, sp - 1u);
+                                return h.pageData(sp);
+                            }
+                            r &= ~(1ull << pos);
+                        }
+                        if(base == bottom)
+                            break;
+                        top = base + static_cast<long long>(k) - 2; // windows overlap by k-1 pages
+                    }
+                }
+                return nullptr;
             }
 
             //! Wave malloc: every lane passes its (unpadded) request size.
@@ -403,11 +494,16 @@ namespace mallocMC
                 const u32 b = dist.collect(h, h.pad(bytesIn));
                 char* result = nullptr;
                 bool want = b != 0u;
-                if(want && b > h.pagesize)
+                // requests above a page: one wave-wide run claim each
+                for(u64 big = hw::ballot(want && b > h.pagesize); big != 0u; big &= big - 1u)
                 {
-                    result = h.allocPageBased<WaveIds>(b);
-                    want = false;
+                    const u32 l = hw::ffs(big) - 1u;
+                    char* r = allocRun(wave::lane(b, l));
+                    if(me == l)
+                        result = r;
                 }
+                if(b > h.pagesize)
+                    want = false;
                 u64 pending = hw::ballot(want);
                 const u32 cls = want ? sizeClass(h, b) : 0xffffffffu;
                 const u32 minAlloc = umax32(b, h.paddedMinChunk);

@@ -1,0 +1,181 @@
+"""Config 3: many independent heaps served by one persistent launch per step.
+
+64 instances; instance i gets floor(1024 * 256**u_i) request slots (u_i
+uniform, so log-uniform in [1024, 262144]) and its own 256 MiB heap.  Each
+step every live slot frees with probability 1/2 and every empty slot
+allocates with probability 1/2 a size drawn from Zipf(1.2) over the powers of
+two 16..65536 B (the steady state keeps about half the slots live).  100
+steps, each one graph replay of mmc_multi_step.  The autotuner sweeps
+pagesize {4K, 16K, 64K} x regionsize {8, 16, 32} x aggregation {1, 16, 64}.
+"""
+from __future__ import annotations
+
+import dataclasses
+import itertools
+import time
+
+import numpy as np
+
+import mallocmc_amd as mmc
+from mallocmc_amd import workloads as wl
+
+MiB = 1024 ** 2
+GiB = 1024 ** 3
+
+
+@dataclasses.dataclass
+class MultiWorkload:
+    slots_per_heap: np.ndarray
+    slot_heap: np.ndarray
+    tiles: np.ndarray
+    table: np.ndarray
+    cdf: np.ndarray
+
+
+def make_workload(n_heaps: int = 64, seed: int = 11, lo: int = 1024, hi: int = 262144, s: float = 1.2):
+    rng = np.random.default_rng(seed)
+    u = rng.random(n_heaps)
+    slots = np.floor(lo * (hi / lo) ** u).astype(np.int64)
+    slot_heap = np.repeat(np.arange(n_heaps, dtype=np.uint32), slots)
+    starts = np.concatenate([[0], np.cumsum(slots)[:-1]])
+    tiles = np.concatenate([np.arange(st, st + n, 64) for st, n in zip(starts, slots)]).astype(np.uint32)
+    table = wl.ZIPF_SIZES
+    w = 1.0 / np.arange(1, len(table) + 1) ** s
+    cdf = np.minimum(np.round(np.cumsum(w / w.sum()) * 2 ** 32), 2 ** 32 - 1).astype(np.uint64)
+    cdf[-1] = 2 ** 32 - 1
+    return MultiWorkload(slots, slot_heap, tiles, table, cdf.astype(np.uint32))
+
+
+class MultiRun:
+    """Heaps, device buffers and the step descriptor of one configuration."""
+
+    def __init__(self, work: MultiWorkload, pagesize: int, regionsize: int, heap_bytes: int = 256 * MiB,
+                 seed: int = 7):
+        import torch
+        self.work = work
+        cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=heap_bytes, regionsize=regionsize,
+                                resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
+        self.allocs = [mmc.ScatterAllocator(cfg, heap_bytes) for _ in range(len(work.slots_per_heap))]
+        self.multi = mmc.MultiHeap(self.allocs)
+        n = len(work.slot_heap)
+        dev = "cuda"
+        self.t = {
+            "slot_heap": torch.from_numpy(work.slot_heap.astype(np.int32)).to(dev),
+            "tiles": torch.from_numpy(work.tiles.astype(np.int32)).to(dev),
+            "table": torch.from_numpy(work.table.astype(np.int32)).to(dev),
+            "cdf": torch.from_numpy(work.cdf.view(np.int32)).to(dev),
+            "offs": torch.full((n,), -1, dtype=torch.int64, device=dev),
+            "sizes": torch.zeros(n, dtype=torch.int32, device=dev),
+            "step": torch.zeros(1, dtype=torch.int64, device=dev),
+            "counters": torch.zeros(4, dtype=torch.int64, device=dev),
+            "cursor": torch.zeros(2, dtype=torch.int32, device=dev),
+        }
+        d = mmc.MmcStepDesc()
+        d.d_slot_heap = self.t["slot_heap"].data_ptr()
+        d.d_tiles = self.t["tiles"].data_ptr()
+        d.n_tiles = len(work.tiles)
+        d.n_slots = n
+        d.d_size_table = self.t["table"].data_ptr()
+        d.d_size_cdf = self.t["cdf"].data_ptr()
+        d.n_sizes = len(work.table)
+        d.p_free_permille = 500
+        d.p_alloc_permille = 500
+        d.seed = seed
+        d.d_offsets = self.t["offs"].data_ptr()
+        d.d_sizes = self.t["sizes"].data_ptr()
+        d.d_step = self.t["step"].data_ptr()
+        d.d_counters = self.t["counters"].data_ptr()
+        d.d_tile_cursor = self.t["cursor"].data_ptr()
+        self.desc = d
+
+    def close(self):
+        self.multi.close()
+        for a in self.allocs:
+            a.close()
+
+    def run(self, steps: int, tuning: mmc.Tuning, graph: bool = True):
+        """Runs `steps` steps; returns (gpu_ms, counters delta)."""
+        import torch
+        c0 = self.t["counters"].cpu().numpy().copy()
+        launch = lambda: self.multi.step(self.desc, tuning)  # noqa: E731
+        if graph:
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    self.multi.step(self.desc, tuning, stream=s)
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+            launch = g.replay
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            launch()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), self.t["counters"].cpu().numpy() - c0
+
+    def check_invariants(self) -> bool:
+        """Live allocations: no overlap within a heap, patterns intact, aligned."""
+        import torch
+        ok = True
+        offs, sizes = self.t["offs"], self.t["sizes"]
+        starts = np.concatenate([[0], np.cumsum(self.work.slots_per_heap)])
+        for h, a in enumerate(self.allocs):
+            o = offs[starts[h]:starts[h + 1]].contiguous()
+            s = sizes[starts[h]:starts[h + 1]].contiguous()
+            live = o >= 0
+            oo = o[live]
+            ss = ((s[live].to(torch.int64) + 15) // 16) * 16
+            order = torch.argsort(oo)
+            oo, ss = oo[order], ss[order]
+            if oo.numel() > 1 and int(((oo[:-1] + ss[:-1]) > oo[1:]).sum()) != 0:
+                ok = False
+            a.write_pattern(o, s, salt=h)
+            r = a.verify_pattern(o, s, salt=h)
+            ok = ok and r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0
+        return ok
+
+
+def run_config3(log=None, steps: int = 100, tune_steps: int = 20, space=None) -> dict:
+    work = make_workload()
+    space = space or dict(pagesize=(4096, 16384, 65536), regionsize=(8, 16, 32), aggregation=(1, 16, 64))
+    points = []
+    t0 = time.time()
+    for P, R in itertools.product(space["pagesize"], space["regionsize"]):
+        for agg in space["aggregation"]:
+            run = MultiRun(work, P, R)
+            tun = mmc.Tuning(aggregation=agg, persistent=True, xcd_slices=8)
+            run.run(3, tun)  # warm up into the steady state
+            ms, dc = run.run(tune_steps, tun)
+            rate = (dc[0] + dc[1]) / (ms / 1e3)
+            points.append(dict(pagesize=P, regionsize=R, aggregation=agg, ops_per_s=float(rate),
+                               null_frac=float(dc[2] / max(1, dc[0]))))
+            run.close()
+            if log:
+                log(f"multi tune {points[-1]}")
+    best = max(points, key=lambda p: p["ops_per_s"])
+    tune_s = time.time() - t0
+    run = MultiRun(work, best["pagesize"], best["regionsize"])
+    tun = mmc.Tuning(aggregation=best["aggregation"], persistent=True, xcd_slices=8)
+    ms, dc = run.run(steps, tun)
+    ok = run.check_invariants()
+    run.close()
+    # untuned reference point: the reference's default geometry, aggregation 1
+    ref = MultiRun(work, 4096, 16)
+    ref_tun = mmc.Tuning(aggregation=1, persistent=True, xcd_slices=0)
+    ref_ms, ref_dc = ref.run(steps, ref_tun)
+    ref.close()
+    out = {
+        "workload": "config 3: 64 heaps x 256 MiB, log-uniform 1024..262144 slots (%d total), Zipf(1.2) sizes "
+                    "16..65536 B, 100 graph-replayed steps of one persistent launch" % len(work.slot_heap),
+        "mallocs_frees_per_s": float((dc[0] + dc[1]) / (ms / 1e3)), "ms_per_step": ms / steps,
+        "mallocs": int(dc[0]), "frees": int(dc[1]), "nulls": int(dc[2]),
+        "tuned": best, "autotune_points": len(points), "autotune_s": tune_s, "invariants_ok": bool(ok),
+        "untuned_4k_r16_agg1": {"mallocs_frees_per_s": float((ref_dc[0] + ref_dc[1]) / (ref_ms / 1e3)),
+                                "ms_per_step": ref_ms / steps, "nulls": int(ref_dc[2])},
+    }
+    if log:
+        log(f"config3: {out}")
+    return out
