@@ -17,8 +17,9 @@ Forms:
 Also reported: the device-atomic ceiling (microbenchmark), the roofline of
 the dominant kernel, the CPU baseline (the oracle on this host's cores, the
 reference's CpuOmp2Blocks execution model with 16-way mutex atomics), the
-serving-loop p50/p99 step latency (config 5, mallocmc_amd/serve.py) and the
-multi-heap batched form (config 3, mallocmc_amd/multiheap.py).
+serving-loop p50/p99 step latency (config 5, mallocmc_amd/serve.py), the
+multi-heap batched form (config 3, mallocmc_amd/multiheap.py) and the
+fragmentation/OOM stress cycles (config 4, mallocmc_amd/oomstress.py).
 """
 from __future__ import annotations
 
@@ -130,6 +131,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-heap (config 3) section")
+    ap.add_argument("--no-oom", action="store_true", help="skip the OOM-stress (config 4) section")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -258,6 +260,11 @@ def main():
         from mallocmc_amd import multiheap
         multi = multiheap.run_config3(log=log, tune_steps=10)
 
+    oom = None
+    if not args.no_oom:
+        from mallocmc_amd import oomstress
+        oom = oomstress.run_config4(log=log)
+
     serve = None
     if not args.no_serve:
         try:
@@ -311,6 +318,7 @@ def main():
         "cpu_baseline": cpu,
         "serve": serve,
         "batched_multi_heap": multi,
+        "oom_stress": oom,
     }
     print(json.dumps(out), flush=True)
 

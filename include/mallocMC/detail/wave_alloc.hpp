@@ -45,6 +45,9 @@ namespace mallocMC
             //! Spread start pages by XCD: each XCD hashes into its own slice of
             //! every access block, so its atomics concentrate on its own lines.
             u32 xcdSlices = 0u;
+            //! Size classes per octave = 2^classSplit (0..3): finer classes waste
+            //! less (chunk <= (1 + 2^-classSplit) x request) but aggregate fewer lanes.
+            u32 classSplit = 0u;
         };
 
         namespace wave
@@ -106,8 +109,9 @@ namespace mallocMC
 
         //! Size class used to group lanes; equal class => one chunk size c
         //! (the class maximum) satisfies b <= c <= max(WF*b, minChunk) for
-        //! every member b.
-        MALLOCMC_D u32 sizeClass(const ScatterHeap& h, u32 b)
+        //! every member b.  Classes are octaves of the request above the minimum
+        //! chunk, each cut into 2^split equal parts.
+        MALLOCMC_D u32 sizeClass(const ScatterHeap& h, u32 b, u32 split)
         {
             if(h.paddedMinChunk != 0u && b <= h.paddedMinChunk)
                 return 0u;
@@ -117,8 +121,10 @@ namespace mallocMC
             const u32 q = (b - 1u) / base;
             if(q == 0u)
                 return 0x80000000u | b; // below the octave base: exact size
-            // octaves above the base: (base*2^(j-1), base*2^j]
-            return 1u + (31u - static_cast<u32>(__builtin_clz(q)));
+            // octave j: q in [2^j, 2^(j+1)), i.e. b in (base*2^j, base*2^(j+1)]
+            const u32 j = 31u - static_cast<u32>(__builtin_clz(q));
+            const u32 sub = j >= split ? ((q - (1u << j)) >> (j - split)) : (q - (1u << j));
+            return 1u + ((j << 8) | sub);
         }
 
         struct WaveAlloc
@@ -505,7 +511,7 @@ namespace mallocMC
                 if(b > h.pagesize)
                     want = false;
                 u64 pending = hw::ballot(want);
-                const u32 cls = want ? sizeClass(h, b) : 0xffffffffu;
+                const u32 cls = want ? sizeClass(h, b, t.classSplit) : 0xffffffffu;
                 const u32 minAlloc = umax32(b, h.paddedMinChunk);
                 while(pending != 0u)
                 {

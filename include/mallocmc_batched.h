@@ -23,6 +23,7 @@ typedef struct mmc_tuning {
     uint32_t xcd_slices;   /* 0/1: reference hash; 8: one slice of each access block per XCD */
     uint32_t persistent;   /* 0: one lane per request; 1: persistent waves pull 64-request tiles */
     uint32_t waves_per_cu; /* persistent launch size (0 = 8) */
+    uint32_t class_split;  /* size classes per octave = 2^class_split (0..3) */
 } mmc_tuning;
 
 int mmc_default_tuning(mmc_tuning* t);
@@ -35,6 +36,27 @@ int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_o
 /* Wave-cooperative counterpart of mmc_free_batch (frees grouped by page). */
 int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, const mmc_tuning* tuning,
                         void* stream);
+
+/* Fill until null: `lanes` lanes (a multiple of 64) each malloc in a loop
+ * until the heap returns null for them, every success appended to
+ * d_offsets / d_sizes at the position taken from *d_count (capacity `cap`;
+ * allocations beyond it are freed again).  Request sizes are a two-part
+ * uniform mixture: with probability p_small_permille/1000 uniform in
+ * [small_lo, small_hi], else uniform in [large_lo, large_hi] (bytes, before
+ * padding), drawn from splitmix64(seed, lane, iteration).  *d_count is not
+ * reset by the call, so fills can append to an existing list. */
+typedef struct mmc_fill_desc {
+    uint64_t seed;
+    uint32_t lanes;
+    uint32_t p_small_permille;
+    uint32_t small_lo, small_hi, large_lo, large_hi;
+    uint64_t cap;
+    uint64_t* d_offsets;
+    uint32_t* d_sizes;
+    uint64_t* d_count;
+} mmc_fill_desc;
+
+int mmc_wave_fill_until_null(mmc_heap* heap, const mmc_fill_desc* desc, const mmc_tuning* tuning, void* stream);
 
 /* ---------------------------------------------------------------- multi-heap */
 typedef struct mmc_multi mmc_multi;

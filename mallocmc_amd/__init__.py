@@ -39,7 +39,7 @@ EXPORTED_SYMBOLS = [
 ]
 BATCHED_SYMBOLS = [
     "mmc_default_tuning", "mmc_wave_malloc_batch", "mmc_wave_free_batch", "mmc_multi_create",
-    "mmc_multi_destroy", "mmc_multi_step",
+    "mmc_multi_destroy", "mmc_multi_step", "mmc_wave_fill_until_null",
     # include/mallocmc_serve.h, bound in mallocmc_amd/serve.py
     "mmc_serve_create", "mmc_serve_destroy", "mmc_serve_load", "mmc_serve_step", "mmc_serve_get_stats",
 ]
@@ -51,20 +51,23 @@ class MmcTuning(ctypes.Structure):
         ("xcd_slices", ctypes.c_uint32),
         ("persistent", ctypes.c_uint32),
         ("waves_per_cu", ctypes.c_uint32),
+        ("class_split", ctypes.c_uint32),
     ]
 
 
 @dataclasses.dataclass
 class Tuning:
     """Wave-path tuning (mmc_tuning): aggregation width 1..64, XCD slices,
-    persistent launch and its waves per CU."""
+    persistent launch and its waves per CU, size classes per octave."""
     aggregation: int = 64
     xcd_slices: int = 0
     persistent: bool = False
     waves_per_cu: int = 8
+    class_split: int = 0
 
     def to_c(self) -> MmcTuning:
-        return MmcTuning(self.aggregation, self.xcd_slices, int(bool(self.persistent)), self.waves_per_cu)
+        return MmcTuning(self.aggregation, self.xcd_slices, int(bool(self.persistent)), self.waves_per_cu,
+                         self.class_split)
 
 
 class MmcStepDesc(ctypes.Structure):

@@ -29,6 +29,7 @@ namespace
         {
             w.aggregation = t->aggregation == 0u ? 64u : (t->aggregation > 64u ? 64u : t->aggregation);
             w.xcdSlices = t->xcd_slices;
+            w.classSplit = t->class_split > 3u ? 3u : t->class_split;
         }
         return w;
     }
@@ -222,6 +223,50 @@ namespace
         return cached[dev];
     }
 
+    //! Fill until null (see mmc_fill_desc): each lane allocates until the heap
+    //! returns null for it; the wave keeps calling the wave malloc until every
+    //! lane stopped.
+    __global__ __launch_bounds__(256) void fillKernel(ScatterHeap h, WaveTuning t, mmc_fill_desc d)
+    {
+        const u32 lane = blockIdx.x * blockDim.x + threadIdx.x;
+        bool active = lane < d.lanes;
+        const WaveAlloc wa{h, t};
+        for(u32 it = 0; hw::ballot(active) != 0u; ++it)
+        {
+            u32 bytes = 0u;
+            if(active)
+            {
+                const u64 r = mix64(d.seed ^ mix64((static_cast<u64>(lane) << 24) + it));
+                const u32 coin = static_cast<u32>(r % 1000u);
+                const u32 u = static_cast<u32>(r >> 32);
+                if(coin < d.p_small_permille)
+                    bytes = d.small_lo + u % (d.small_hi - d.small_lo + 1u);
+                else
+                    bytes = d.large_lo + u % (d.large_hi - d.large_lo + 1u);
+            }
+            void* p = wa.malloc<NoopDistribution>(bytes);
+            if(active)
+            {
+                if(p == nullptr)
+                    active = false;
+                else
+                {
+                    const u64 pos = atom::add(d.d_count, u64{1});
+                    if(pos < d.cap)
+                    {
+                        d.d_offsets[pos] = h.offsetOf(p);
+                        d.d_sizes[pos] = bytes;
+                    }
+                    else
+                    {
+                        h.destroy<WaveIds>(p);
+                        active = false;
+                    }
+                }
+            }
+        }
+    }
+
     unsigned persistentBlocks(const mmc_tuning* t)
     {
         const int cus = cuCount();
@@ -246,6 +291,7 @@ int mmc_default_tuning(mmc_tuning* t)
     t->xcd_slices = 0u;
     t->persistent = 0u;
     t->waves_per_cu = 8u;
+    t->class_split = 0u;
     return MMC_OK;
 }
 
@@ -302,6 +348,18 @@ int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, c
         hipLaunchKernelGGL(waveFreeKernel, dim3(mmc::gridFor(n, 256)), dim3(256), 0, s, heap->view, t, d_offsets, n);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "wave free launch");
+}
+
+int mmc_wave_fill_until_null(mmc_heap* heap, const mmc_fill_desc* desc, const mmc_tuning* tuning, void* stream)
+{
+    if(!heap || !desc || !desc->d_offsets || !desc->d_sizes || !desc->d_count || desc->lanes == 0
+       || desc->small_hi < desc->small_lo || desc->large_hi < desc->large_lo || desc->p_small_permille > 1000u)
+        return mmc::fail(MMC_EINVAL, "bad fill descriptor");
+    const WaveTuning t = toTuning(tuning);
+    hipLaunchKernelGGL(fillKernel, dim3(mmc::gridFor(desc->lanes, 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), heap->view, t, *desc);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "fill launch");
 }
 
 int mmc_multi_create(mmc_heap* const* heaps, uint32_t nheaps, mmc_multi** out)

@@ -1,0 +1,127 @@
+"""Config 4: fragmentation / OOM stress.
+
+Heap 64 GiB at pagesize 64 KiB.  2,097,152 lanes allocate bimodal sizes (80%
+uniform 16..256 B, 20% uniform 8..60 KiB) until the heap returns null for
+each of them (verify_heap's AllocAll loop, tests/verify_heap.cpp:333-361 of
+the reference), then a seeded random half of all allocations is freed and the
+lanes refill; 5 cycles.  After every fill: no two live allocations overlap,
+every allocation holds its pattern and is aligned and inside the page area;
+the fill fraction is reported.  After the last cycle everything is freed and
+getAvailableSlots must return to its initial values.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+import mallocmc_amd as mmc
+
+GiB = 1024 ** 3
+MiB = 1024 ** 2
+
+
+class MmcFillDesc(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("lanes", ctypes.c_uint32),
+        ("p_small_permille", ctypes.c_uint32),
+        ("small_lo", ctypes.c_uint32),
+        ("small_hi", ctypes.c_uint32),
+        ("large_lo", ctypes.c_uint32),
+        ("large_hi", ctypes.c_uint32),
+        ("cap", ctypes.c_uint64),
+        ("d_offsets", ctypes.c_void_p),
+        ("d_sizes", ctypes.c_void_p),
+        ("d_count", ctypes.c_void_p),
+    ]
+
+
+def _bind():
+    L = mmc.lib()
+    if not getattr(L, "_fill_bound", False):
+        L.mmc_wave_fill_until_null.argtypes = [ctypes.c_void_p, ctypes.POINTER(MmcFillDesc),
+                                               ctypes.POINTER(mmc.MmcTuning), ctypes.c_void_p]
+        L.mmc_wave_fill_until_null.restype = ctypes.c_int
+        L._fill_bound = True
+    return L
+
+
+def _no_overlap(torch, offs, sizes):
+    o = offs
+    s = ((sizes.to(torch.int64) + 15) // 16) * 16
+    order = torch.argsort(o)
+    o, s = o[order], s[order]
+    if o.numel() < 2:
+        return True
+    return int(((o[:-1] + s[:-1]) > o[1:]).sum().item()) == 0
+
+
+def run_config4(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles: int = 5, pagesize: int = 65536,
+                regionsize: int = 16, tuning: mmc.Tuning | None = None, log=None, cap: int | None = None) -> dict:
+    import torch
+    L = _bind()
+    tuning = tuning or mmc.Tuning(aggregation=64, class_split=2)
+    cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=min(2 * GiB, heap_bytes), regionsize=regionsize,
+                            resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
+    alloc = mmc.ScatterAllocator(cfg, heap_bytes)
+    g = alloc.geometry()
+    page_bytes = int(g.numpages) * pagesize
+    probe = (16, 256, 8192, pagesize, 4 * pagesize)
+    slots0 = [alloc.getAvailableSlots(s) for s in probe]
+    cap = cap or max(1 << 20, heap_bytes // 4096)
+    offs = torch.full((cap,), -1, dtype=torch.int64, device="cuda")
+    sizes = torch.zeros(cap, dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    rng = np.random.default_rng(4)
+    results = []
+    ok = True
+    for cyc in range(cycles):
+        d = MmcFillDesc(seed=1000 + cyc, lanes=lanes, p_small_permille=800, small_lo=16, small_hi=256,
+                        large_lo=8 * 1024, large_hi=60 * 1024, cap=cap, d_offsets=offs.data_ptr(),
+                        d_sizes=sizes.data_ptr(), d_count=count.data_ptr())
+        before = int(count.item())
+        t0 = time.perf_counter()
+        mmc._check(L.mmc_wave_fill_until_null(alloc.handle, ctypes.byref(d), ctypes.byref(tuning.to_c()),
+                                              mmc._stream_ptr()), "fill")
+        torch.cuda.synchronize()
+        t_fill = time.perf_counter() - t0
+        n = min(int(count.item()), cap)
+        live_o, live_s = offs[:n], sizes[:n]
+        overlap_ok = _no_overlap(torch, live_o, live_s)
+        alloc.write_pattern(live_o, live_s, salt=cyc << 48)
+        v = alloc.verify_pattern(live_o, live_s, salt=cyc << 48)
+        cyc_ok = overlap_ok and v["mismatches"] == 0 and v["misaligned"] == 0 and v["outside"] == 0 and v["nulls"] == 0
+        ok = ok and cyc_ok
+        live_bytes = int((((live_s.to(torch.int64) + 15) // 16) * 16).sum().item())
+        slots = [alloc.getAvailableSlots(s) for s in probe]
+        # free a seeded random half, keep the rest at the front of the list
+        mask = torch.from_numpy(rng.random(n) < 0.5).cuda()
+        t1 = time.perf_counter()
+        alloc.wave_free_batch(offs[:n][mask].contiguous(), tuning=tuning)
+        torch.cuda.synchronize()
+        t_free = time.perf_counter() - t1
+        keep = ~mask
+        m = int(keep.sum().item())
+        offs[:m] = offs[:n][keep].clone()
+        sizes[:m] = sizes[:n][keep].clone()
+        count.fill_(m)
+        rec = dict(cycle=cyc, new_allocations=n - before, live_allocations=n, fill_fraction=live_bytes / page_bytes,
+                   fill_s=t_fill, free_s=t_free, invariants_ok=bool(cyc_ok), slots_after_fill=slots)
+        results.append(rec)
+        if log:
+            log(f"config4 cycle {rec}")
+    # free everything that is left; the slot counts come back
+    n = int(count.item())
+    alloc.wave_free_batch(offs[:n].contiguous(), tuning=tuning)
+    torch.cuda.synchronize()
+    slots_end = [alloc.getAvailableSlots(s) for s in probe]
+    alloc.close()
+    return {
+        "workload": f"config 4: heap {heap_bytes // GiB} GiB, pagesize {pagesize // 1024} KiB, {lanes} lanes fill "
+                    f"until null with 80% U[16,256] B / 20% U[8,60] KiB, free a random half, {cycles} cycles",
+        "cycles": results, "invariants_ok": bool(ok), "slots_restored": slots_end == slots0,
+        "mean_fill_fraction": float(np.mean([r["fill_fraction"] for r in results])),
+        "tuning": dict(aggregation=tuning.aggregation, class_split=tuning.class_split),
+    }

@@ -1,0 +1,19 @@
+"""GPU test of the fragmentation / OOM stress cycle (config 4 shape on a
+1 GiB heap): fill until null, free a random half, refill; invariants after
+every fill and the slot counts restored at the end."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GiB = 1024 ** 3
+
+
+@pytest.mark.parametrize("split", [0, 2])
+def test_oom_stress_cycles(mmc, cuda, split):
+    from mallocmc_amd import oomstress
+    r = oomstress.run_config4(heap_bytes=1 * GiB, lanes=65536, cycles=3,
+                              tuning=mmc.Tuning(aggregation=64, class_split=split), log=print)
+    assert r["invariants_ok"], r
+    assert r["slots_restored"], r
+    for c in r["cycles"]:
+        assert c["fill_fraction"] > 0.5, c
