@@ -38,7 +38,9 @@ int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, c
                         void* stream);
 
 /* Fill until null: `lanes` lanes (a multiple of 64) each malloc in a loop
- * until the heap returns null for them, every success appended to
+ * until the heap returns null for them -- or, with d_stop set, until the
+ * first null any lane sees (it sets *d_stop, which every wave checks before
+ * its next request) -- every success appended to
  * d_offsets / d_sizes at the position taken from *d_count (capacity `cap`;
  * allocations beyond it are freed again).  Request sizes are a two-part
  * uniform mixture: with probability p_small_permille/1000 uniform in
@@ -54,6 +56,7 @@ typedef struct mmc_fill_desc {
     uint64_t* d_offsets;
     uint32_t* d_sizes;
     uint64_t* d_count;
+    uint32_t* d_stop; /* optional: stop everyone at the first null (not reset by the call) */
 } mmc_fill_desc;
 
 int mmc_wave_fill_until_null(mmc_heap* heap, const mmc_fill_desc* desc, const mmc_tuning* tuning, void* stream);

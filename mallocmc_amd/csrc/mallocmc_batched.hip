@@ -233,6 +233,14 @@ namespace
         const WaveAlloc wa{h, t};
         for(u32 it = 0; hw::ballot(active) != 0u; ++it)
         {
+            if(d.d_stop)
+            {
+                u32 stop = 0u;
+                if(hw::laneid() == 0u)
+                    stop = atom::load(d.d_stop);
+                if(__builtin_amdgcn_readfirstlane(stop) != 0u)
+                    break;
+            }
             u32 bytes = 0u;
             if(active)
             {
@@ -248,7 +256,11 @@ namespace
             if(active)
             {
                 if(p == nullptr)
+                {
                     active = false;
+                    if(d.d_stop && bytes != 0u)
+                        atom::store(d.d_stop, 1u);
+                }
                 else
                 {
                     const u64 pos = atom::add(d.d_count, u64{1});

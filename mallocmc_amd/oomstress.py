@@ -1,10 +1,11 @@
 """Config 4: fragmentation / OOM stress.
 
 Heap 64 GiB at pagesize 64 KiB.  2,097,152 lanes allocate bimodal sizes (80%
-uniform 16..256 B, 20% uniform 8..60 KiB) until the heap returns null for
-each of them (verify_heap's AllocAll loop, tests/verify_heap.cpp:333-361 of
-the reference), then a seeded random half of all allocations is freed and the
-lanes refill; 5 cycles.  After every fill: no two live allocations overlap,
+uniform 16..256 B, 20% uniform 8..60 KiB) until the first null any of them
+sees (default), or until the heap returns null for each of them
+(first_null=False: verify_heap's AllocAll loop, tests/verify_heap.cpp:333-361
+of the reference, where every lane pays a failing full-heap scan), then a
+seeded random half of all allocations is freed and the lanes refill; 5 cycles.  After every fill: no two live allocations overlap,
 every allocation holds its pattern and is aligned and inside the page area;
 the fill fraction is reported.  After the last cycle everything is freed and
 getAvailableSlots must return to its initial values.
@@ -35,6 +36,7 @@ class MmcFillDesc(ctypes.Structure):
         ("d_offsets", ctypes.c_void_p),
         ("d_sizes", ctypes.c_void_p),
         ("d_count", ctypes.c_void_p),
+        ("d_stop", ctypes.c_void_p),
     ]
 
 
@@ -59,7 +61,11 @@ def _no_overlap(torch, offs, sizes):
 
 
 def run_config4(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles: int = 5, pagesize: int = 65536,
-                regionsize: int = 16, tuning: mmc.Tuning | None = None, log=None, cap: int | None = None) -> dict:
+                regionsize: int = 16, tuning: mmc.Tuning | None = None, log=None, cap: int | None = None,
+                first_null: bool = True) -> dict:
+    """first_null: every lane stops at the first null any lane sees (the
+    heap's fill at its first failure); else each lane allocates until its own
+    null (verify_heap's AllocAll)."""
     import torch
     L = _bind()
     tuning = tuning or mmc.Tuning(aggregation=64, class_split=2)
@@ -74,13 +80,16 @@ def run_config4(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles
     offs = torch.full((cap,), -1, dtype=torch.int64, device="cuda")
     sizes = torch.zeros(cap, dtype=torch.int32, device="cuda")
     count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    stop = torch.zeros(1, dtype=torch.int32, device="cuda")
     rng = np.random.default_rng(4)
     results = []
     ok = True
     for cyc in range(cycles):
         d = MmcFillDesc(seed=1000 + cyc, lanes=lanes, p_small_permille=800, small_lo=16, small_hi=256,
                         large_lo=8 * 1024, large_hi=60 * 1024, cap=cap, d_offsets=offs.data_ptr(),
-                        d_sizes=sizes.data_ptr(), d_count=count.data_ptr())
+                        d_sizes=sizes.data_ptr(), d_count=count.data_ptr(),
+                        d_stop=stop.data_ptr() if first_null else None)
+        stop.zero_()
         before = int(count.item())
         t0 = time.perf_counter()
         mmc._check(L.mmc_wave_fill_until_null(alloc.handle, ctypes.byref(d), ctypes.byref(tuning.to_c()),
@@ -120,7 +129,8 @@ def run_config4(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles
     alloc.close()
     return {
         "workload": f"config 4: heap {heap_bytes // GiB} GiB, pagesize {pagesize // 1024} KiB, {lanes} lanes fill "
-                    f"until null with 80% U[16,256] B / 20% U[8,60] KiB, free a random half, {cycles} cycles",
+                    f"until {'the first null' if first_null else 'each lane sees a null'} with 80% U[16,256] B / "
+                    f"20% U[8,60] KiB, free a random half, {cycles} cycles",
         "cycles": results, "invariants_ok": bool(ok), "slots_restored": slots_end == slots0,
         "mean_fill_fraction": float(np.mean([r["fill_fraction"] for r in results])),
         "tuning": dict(aggregation=tuning.aggregation, class_split=tuning.class_split),
