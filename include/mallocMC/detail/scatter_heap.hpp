@@ -89,6 +89,9 @@ namespace mallocMC
             u32 hierarchyThreshold = 0;
             u32 minChunkSize = 0;
             u32 maxOnPageMasks = 0;
+            //! 32-bit segment-mask words per page in the side mask array
+            //! (at most 32: one 128-byte line per page).
+            u32 sideMaskWords = 0;
             u32 clampedBlocks = 0;
             u64 ptesOffset = 0;
             u64 regionsOffset = 0;
@@ -131,6 +134,14 @@ namespace mallocMC
             const u32 minSegment = 32u * g.minChunkSize + 4u;
             const u32 onPage = g.minChunkSize > g.hierarchyThreshold ? 0u : (P + (minSegment - 1u)) / minSegment;
             g.maxOnPageMasks = umin32(32u, onPage);
+            // segments of the smallest chunk a page can hold bound the mask
+            // words any page needs (fewer for every larger chunk); padded
+            // requests are multiples of dataAlignment
+            {
+                const u32 align = c.dataAlignment - 1u;
+                const u32 cmin = umax32((g.minChunkSize + align) & ~align, umax32(c.dataAlignment, 1u));
+                g.sideMaskWords = umin32(32u, P / (32u * cmin + 4u) + 1u);
+            }
 
             const u64 R = c.regionsize;
             u32 numregions = static_cast<u32>(memsize / (R * (12u + static_cast<u64>(P)) + 4u));
@@ -228,6 +239,7 @@ namespace mallocMC
             char* pages = nullptr;
             u32* ptes = nullptr;
             u32* regions = nullptr;
+            u32* masks = nullptr; // side segment masks, maskStride words per page
             ScatterScalars* scalars = nullptr;
             u64 memsize = 0;
             u32 pagesize = 0;
@@ -238,6 +250,7 @@ namespace mallocMC
             u32 hierarchyThreshold = 0;
             u32 minChunkSize = 0;
             u32 maxOnPageMasks = 0;
+            u32 maskStride = 0;
             u32 paddedMinChunk = 0;
             u32 alignMask = 0;
             u32 hashingK = 0;
@@ -271,10 +284,15 @@ namespace mallocMC
             {
                 return static_cast<u32>(pageShift ? (offset >> pageShift) : (offset / pagesize));
             }
-            //! onPageMasksPosition (Scatter.hpp:315-318)
-            MALLOCMC_D u32* onPageMasks(u32 p, u32 nMasks) const
+            //! Segment masks of a hierarchical page: word s is segment s's
+            //! 32-chunk mask.  The reference keeps them in the page's last
+            //! 4*nMasks bytes (onPageMasksPosition, Scatter.hpp:315-318); here
+            //! they sit in a side array, one 128-byte line per page, so no
+            //! payload byte ever aliases allocator metadata.  Chunk placement
+            //! is unchanged: chunks still end where the on-page masks would begin.
+            MALLOCMC_D u32* onPageMasks(u32 p, u32 /*nMasks*/) const
             {
-                return reinterpret_cast<u32*>(pageData(p) + pagesize - 4u * nMasks);
+                return masks + static_cast<u64>(p) * maskStride;
             }
             //! Shrink::applyPadding (Shrink.hpp:132-137)
             MALLOCMC_HD u32 pad(u32 bytes) const
@@ -480,11 +498,17 @@ namespace mallocMC
             }
 
             //! Page::init (Scatter.hpp:249-258): clear the on-page mask area.
-            MALLOCMC_D void pageInit(u32 page) const
+            //! Clears a page's segment masks (the reference's page reset zeroes
+            //! the on-page masks, Scatter.hpp:691-697, 879-881).  Returning
+            //! exchanges: the caller may consume the result to order later
+            //! atomics after these have been performed.
+            MALLOCMC_D u32 pageInit(u32 page) const
             {
-                u32* w = reinterpret_cast<u32*>(pageData(page) + pagesize - 4u * maxOnPageMasks);
-                for(u32 i = 0; i < maxOnPageMasks; ++i)
-                    atom::store(&w[i], 0u);
+                u32* w = masks + static_cast<u64>(page) * maskStride;
+                u32 acc = 0u;
+                for(u32 i = 0; i < maskStride; ++i)
+                    acc |= atom::exch(&w[i], 0u);
+                return acc;
             }
 
             //! deallocChunked (Scatter.hpp:646-736)
@@ -730,10 +754,13 @@ namespace mallocMC
             const ScatterGeometry& g,
             void* pool,
             u64 memsize,
-            ScatterScalars* scalars)
+            ScatterScalars* scalars,
+            u32* sideMasks)
         {
             ScatterHeap h;
             h.pages = static_cast<char*>(pool);
+            h.masks = sideMasks;
+            h.maskStride = g.sideMaskWords;
             h.ptes = reinterpret_cast<u32*>(h.pages + g.ptesOffset);
             h.regions = reinterpret_cast<u32*>(h.pages + g.regionsOffset);
             h.scalars = scalars;

@@ -650,19 +650,38 @@ namespace mallocMC
                 bool locked = false;
                 if(pageLeader && old == pageK)
                     locked = atom::cas(h.pteCount(page), 0u, P) == 0u;
-                if(hw::ballot(locked) == 0u)
+                const u64 lockedLanes = hw::ballot(locked);
+                if(lockedLanes == 0u)
                     return;
                 // order as Scatter.hpp:691-710: clear masks, drop the chunk size,
                 // unlock (the count lock keeps allocators out meanwhile).  The
-                // reference's full device fence before the unlock is a release:
-                // the unlock carries it, which drops the L2 invalidate a
-                // sequentially consistent fence adds on gfx950.
-                if(locked)
+                // reference fences the device before the unlock so that the
+                // mask stores are seen first.  Here every write of the reset is
+                // a returning atomic on metadata that only atomics touch (the
+                // side mask array and the PTE), so waiting for their results
+                // before the unlock is enough; no L2 write-back is needed.
+                // Only a hierarchical page has segment masks; the wave clears
+                // each locked page's words with one vector exchange (lane s
+                // takes word s of the page's 128-byte line).
+                u32 nseg = 0u;
+                if(locked && hier)
                 {
-                    h.pageInit(page);
-                    atom::cas(h.pteChunksize(page), cs, 0u);
-                    atom::sub_release(h.pteCount(page), P);
+                    const u32 segmentsize = cs * 32u + 4u;
+                    const u32 full = umin32(32u, P / segmentsize);
+                    nseg = full + (h.calcAdditional(full, segmentsize, cs) > 0u ? 1u : 0u);
                 }
+                u32 z = 0u;
+                for(u64 m = lockedLanes; m != 0u; m &= m - 1u)
+                {
+                    const u32 l = hw::ffs(m) - 1u;
+                    const u32 lpage = wave::lane(page, l);
+                    if(me < wave::lane(nseg, l))
+                        z |= atom::exch(h.onPageMasks(lpage, 0u) + me, 0u);
+                }
+                const u32 oc = locked ? atom::cas(h.pteChunksize(page), cs, 0u) : 0u;
+                hw::consume(z ^ oc);
+                if(locked)
+                    atom::sub(h.pteCount(page), P);
             }
         };
     } // namespace detail

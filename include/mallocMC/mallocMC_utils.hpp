@@ -94,6 +94,20 @@ namespace mallocMC
             const std::uint32_t hi = __shfl(static_cast<std::uint32_t>(v >> 32), static_cast<int>(lane), 64);
             return (static_cast<std::uint64_t>(hi) << 32) | lo;
         }
+
+        //! Makes the wave wait for `v` (the result of returning atomics) before
+        //! any later memory operation is issued: the empty asm reads the
+        //! register, so the compiler must wait for the data, and its memory
+        //! clobber keeps later atomics from being moved above it.  A returning
+        //! device-scope atomic has been performed at the device's coherence
+        //! point once its result is back, so every atomic issued after this
+        //! point is ordered after it for all observers that use atomics on
+        //! those words.  No L2 write-back is involved (an agent-scope release
+        //! on gfx950 writes back the whole L2 of the issuing XCD).
+        MALLOCMC_D void consume(std::uint32_t v)
+        {
+            asm volatile("" ::"v"(v) : "memory");
+        }
     } // namespace hw
 
     //! Agent-scope relaxed atomics on heap metadata (the reference uses

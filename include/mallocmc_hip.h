@@ -70,6 +70,8 @@ typedef struct mmc_geometry {
     uint64_t ptesOffset;
     uint64_t regionsOffset;
     uint64_t usedBytes;
+    uint32_t sideMaskWords; /* segment-mask words per page in the side array (no reference counterpart) */
+    uint32_t reserved0;
 } mmc_geometry;
 
 typedef struct mmc_heap mmc_heap;
@@ -85,7 +87,10 @@ int mmc_compute_geometry(const mmc_config* cfg, uint64_t bytes, mmc_geometry* ou
 
 /* Allocator ctor (src/include/mallocMC/allocator.hpp:136-159, 179-183):
  * reserves the pool, aligns it (Shrink.hpp:109-130) and enqueues the heap
- * init kernel (Scatter.hpp:1057-1096) on `stream`. */
+ * init kernel (Scatter.hpp:1057-1096) on `stream`.  Besides the pool the heap
+ * holds a side array of segment masks (up to 128 B per page, at most 0.8% of
+ * the pool for pagesize >= 16 KiB), where the reference keeps them in the last
+ * bytes of each page; page geometry and offsets are the reference's. */
 int mmc_create(const mmc_config* cfg, uint64_t bytes, void* stream, mmc_heap** out);
 
 /* Heap inside a pool the caller reserved (ReservePool policy,

@@ -122,6 +122,8 @@ class MmcGeometry(ctypes.Structure):
         ("ptesOffset", ctypes.c_uint64),
         ("regionsOffset", ctypes.c_uint64),
         ("usedBytes", ctypes.c_uint64),
+        ("sideMaskWords", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
     ]
 
 

@@ -63,6 +63,8 @@ namespace mmc
         o->ptesOffset = g.ptesOffset;
         o->regionsOffset = g.regionsOffset;
         o->usedBytes = g.usedBytes;
+        o->sideMaskWords = g.sideMaskWords;
+        o->reserved0 = 0u;
     }
 
     unsigned gridFor(u64 n, unsigned threads)
@@ -78,7 +80,7 @@ using mmc::hipFail;
 // ------------------------------------------------------------------ kernels
 namespace
 {
-    //! Heap init (Scatter.hpp:1026-1048): PTEs, page mask tails and region
+    //! Heap init (Scatter.hpp:1026-1048): PTEs, segment masks and region
     //! fill levels to zero, scalars to their start values.
     __global__ void initKernel(ScatterHeap h)
     {
@@ -89,10 +91,10 @@ namespace
             h.ptes[3u * i] = 0u;
             h.ptes[3u * i + 1u] = 0u;
             h.ptes[3u * i + 2u] = 0u;
-            u32* w = reinterpret_cast<u32*>(h.pages + (i + 1u) * h.pagesize - 4u * h.maxOnPageMasks);
-            for(u32 k = 0; k < h.maxOnPageMasks; ++k)
-                w[k] = 0u;
         }
+        const u64 maskWords = static_cast<u64>(h.numpages) * h.maskStride;
+        for(u64 i = tid; i < maskWords; i += stride)
+            h.masks[i] = 0u;
         for(u64 i = tid; i < h.numregions; i += stride)
             h.regions[i] = 0u;
         if(tid == 0)
@@ -280,11 +282,20 @@ namespace mmc
         err = hipMalloc(&heap->scalars, sizeof(ScatterScalars));
         if(err == hipSuccess)
             err = hipMalloc(&heap->scratch, sizeof(u32) * (4u * kRunThreads + 64u));
+        if(err == hipSuccess)
+            err = hipMalloc(&heap->masks, sizeof(u32) * static_cast<u64>(g.numpages) * g.sideMaskWords);
         if(err != hipSuccess)
         {
             if(heap->ownsPool)
                 (void) hipFree(raw);
             heap->rawPool = nullptr;
+            if(heap->scalars)
+                (void) hipFree(heap->scalars);
+            if(heap->scratch)
+                (void) hipFree(heap->scratch);
+            heap->scalars = nullptr;
+            heap->scratch = nullptr;
+            heap->masks = nullptr;
             return hipFail(err, "hipMalloc(heap metadata)");
         }
         // slot-count output and the persistent-kernel work cursor start at zero
@@ -299,8 +310,8 @@ namespace mmc
         heap->pool = pool;
         heap->memsize = memsize;
         heap->geom = g;
-        heap->view = makeHeapView(cfg, g, pool, memsize, heap->scalars);
-        // init kernel: 140 B per page, HBM-bound; fill the chip
+        heap->view = makeHeapView(cfg, g, pool, memsize, heap->scalars, heap->masks);
+        // init kernel: 12 B PTE + 4 B per side mask word per page, HBM-bound; fill the chip
         const unsigned threads = 256;
         u64 blocks64 = (g.numpages + threads - 1u) / threads;
         const unsigned blocks = static_cast<unsigned>(blocks64 < 8192u ? (blocks64 ? blocks64 : 1u) : 8192u);
@@ -319,9 +330,12 @@ namespace mmc
             (void) hipFree(heap->scalars);
         if(heap->scratch)
             (void) hipFree(heap->scratch);
+        if(heap->masks)
+            (void) hipFree(heap->masks);
         heap->rawPool = nullptr;
         heap->scalars = nullptr;
         heap->scratch = nullptr;
+        heap->masks = nullptr;
         heap->pool = nullptr;
         heap->memsize = 0;
     }

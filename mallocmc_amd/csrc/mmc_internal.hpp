@@ -20,6 +20,7 @@ struct mmc_heap
     std::uint64_t memsize = 0;
     mallocMC::detail::ScatterScalars* scalars = nullptr;
     std::uint32_t* scratch = nullptr; // slot-count staging
+    std::uint32_t* masks = nullptr;   // side segment masks (geom.sideMaskWords per page)
     mallocMC::detail::ScatterGeometry geom{};
     mallocMC::detail::ScatterHeap view{};
 };

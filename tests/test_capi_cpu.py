@@ -64,7 +64,33 @@ def test_host_geometry_equals_oracle(mmc, oracle_mod, pagesize, ab, region, heap
     g = mmc.compute_geometry(cfg, heap)
     og = oracle_mod.geometry(oracle_mod.make_config(pagesize=pagesize, accessblocksize=ab, regionsize=region), heap)
     for f, _ in mmc.MmcGeometry._fields_:
+        if f in ("sideMaskWords", "reserved0"):  # side mask array: no reference counterpart
+            continue
         assert getattr(g, f) == getattr(og, f), f
+
+
+@pytest.mark.parametrize("pagesize", [128, 256, 1000, 1024, 4096, 8192, 16384, 65536, 1 << 20])
+@pytest.mark.parametrize("align", [1, 4, 16, 64])
+def test_side_mask_words_cover_every_hierarchical_chunk(mmc, oracle_mod, pagesize, align):
+    """Brute force over every chunk size a page can take: the segments it
+    needs (full segments plus a partial one, Scatter.hpp:365-383, 494-510)
+    never exceed the side array's words per page, and the smallest chunk
+    needs all of them or one fewer."""
+    cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=64 * MiB, regionsize=16, dataAlignment=align)
+    g = mmc.compute_geometry(cfg, 64 * MiB)
+    og = oracle_mod.geometry(oracle_mod.make_config(pagesize=pagesize, accessblocksize=64 * MiB, regionsize=16), 64 * MiB)
+    ht, mc = og.hierarchyThreshold, og.minChunkSize
+    assert 1 <= g.sideMaskWords <= 32
+    cmin = max(-(-mc // align) * align, align)
+    most = 0
+    for c in range(cmin, ht + 1, align):
+        seg = 32 * c + 4
+        full = min(32, pagesize // seg)
+        add = 0 if full == 32 else max(0, pagesize - full * seg - 4) // c
+        most = max(most, full + (1 if add > 0 else 0))
+    assert most <= g.sideMaskWords
+    if most:
+        assert g.sideMaskWords - most <= 1
 
 
 def test_invalid_config_is_rejected(mmc):
