@@ -132,54 +132,141 @@ namespace mallocMC
             const ScatterHeap& h;
             WaveTuning t;
 
-            //! Claims up to popc(mask) chunks of size cs on `page` for the group
+            //! Where the chunk bits of a page of chunk size cs live: the PTE
+            //! bitmask (flat pages) or the page's segment masks (hierarchical).
+            struct Layout
+            {
+                u32* words;
+                u32 nWords;
+                u32 full;
+                u32 additional;
+                bool hier;
+            };
+
+            MALLOCMC_D Layout layout(u32 page, u32 cs) const
+            {
+                Layout L;
+                L.hier = cs <= h.hierarchyThreshold;
+                if(L.hier)
+                {
+                    const u32 segmentsize = cs * 32u + 4u;
+                    L.full = umin32(32u, h.pagesize / segmentsize);
+                    L.additional = h.calcAdditional(L.full, segmentsize, cs);
+                    L.nWords = L.full + (L.additional > 0u ? 1u : 0u);
+                    L.words = h.onPageMasks(page, L.nWords);
+                }
+                else
+                {
+                    L.full = 0u;
+                    L.additional = 0u;
+                    L.nWords = 1u;
+                    L.words = h.pteBitmask(page);
+                }
+                return L;
+            }
+
+            //! One lane: resets an empty page -- lock (count 0 -> pagesize),
+            //! clear its segment masks, drop its chunk size, unlock -- if no
+            //! one holds a reservation on it.  Orders the unlock after the
+            //! other writes as the wave free does.
+            MALLOCMC_D void resetEmpty(u32 page) const
+            {
+                const u32 P = h.pagesize;
+                if(atom::cas(h.pteCount(page), 0u, P) != 0u)
+                    return;
+                const u32 cs = atom::load(h.pteChunksize(page));
+                u32 z = 0u;
+                if(cs != 0u && cs <= h.hierarchyThreshold)
+                {
+                    const Layout L = layout(page, cs);
+                    for(u32 w = 0; w < L.nWords; ++w)
+                        z |= atom::exch(&L.words[w], 0u);
+                }
+                const u32 oc = cs != 0u ? atom::cas(h.pteChunksize(page), cs, 0u) : 0u;
+                hw::consume(z ^ oc);
+                atom::sub(h.pteCount(page), P);
+            }
+
+            //! One lane: returns `k` reserved slots of `page`.  With
+            //! resetfreedpages, a return that leaves the count at zero means
+            //! no chunk of the page is in use and no one else holds a
+            //! reservation, so the page is reset as the last free would have
+            //! (a reservation of a claim that backs off can hide the empty
+            //! count from that free).
+            MALLOCMC_D void backOff(u32 page, u32 k) const
+            {
+                const u32 old = atom::sub(h.pteCount(page), k);
+                if(h.resetfreedpages && old == k)
+                    resetEmpty(page);
+            }
+
+            //! Claims up to popc(need) chunks of size cs on `page` for the group
             //! lanes in `need` (a subset of the wave).  Returns the lanes served;
-            //! each served lane's `out` gets its chunk.  `formatted`: this wave
-            //! just set the page's chunk size from 0.
-            MALLOCMC_D u64 claimOnPage(u32 page, u32 cs, u32 bmax, u32 maxcs, u64 need, bool formatted,
+            //! each served lane's `out` gets its chunk.  `fresh`: the scan saw
+            //! the page unformatted, so the claim formats it (CAS 0 -> cs).
+            //!
+            //! The reference's tryUsePage (Scatter.hpp:469-517) reserves with
+            //! atomicAdd(count), then re-checks the chunk size with a CAS, then
+            //! reads the mask word it will claim from.  All three are issued
+            //! in one round trip here: the mask snapshot is only a hint (the
+            //! atomicOr's result decides which chunks are won), and the CAS and
+            //! the add touch different words, so their order does not matter
+            //! to any outcome the reference can reach.
+            MALLOCMC_D u64 claimOnPage(u32 page, u32 cs, u32 bmax, u32 maxcs, u64 need, bool fresh,
                                        char*& out) const
             {
                 const u32 me = hw::laneid();
                 const u32 P = h.pagesize;
+                const bool recheck = h.resetfreedpages || fresh;
+                bool formatted = false;
                 for(;;)
                 {
                     const u32 k = hw::popc(need);
-                    u32 old = 0u;
+                    u32 old = 0u, oldcs = 0u;
                     if(me == 0u)
-                        old = atom::add(h.pteCount(page), k);
-                    old = wave::lane(old, 0u);
-                    if(h.resetfreedpages)
                     {
-                        if(old >= P)
-                        {
-                            // page locked by a concurrent reset
-                            if(me == 0u)
-                                atom::sub(h.pteCount(page), k);
-                            if(formatted)
-                            {
-                                // D5 of scatter_heap.hpp, for the wave
-                                if(me == 0u)
-                                    while(atom::load(h.pteCount(page)) >= P)
-                                        __builtin_amdgcn_s_sleep(1);
-                                __builtin_amdgcn_wave_barrier();
-                                continue;
-                            }
-                            return 0u;
-                        }
-                        u32 oldcs = 0u;
-                        if(me == 0u)
+                        old = atom::add(h.pteCount(page), k);
+                        if(recheck)
                             oldcs = atom::cas(h.pteChunksize(page), 0u, cs);
-                        oldcs = wave::lane(oldcs, 0u);
-                        if(oldcs != 0u && oldcs != cs)
+                    }
+                    Layout L = layout(page, cs);
+                    u32 snap = 0u;
+                    if(me < L.nWords)
+                        snap = atom::load(&L.words[me]);
+                    old = wave::lane(old, 0u);
+                    oldcs = recheck ? wave::lane(oldcs, 0u) : cs;
+                    if(oldcs == 0u)
+                        formatted = true; // this wave (re)formatted the page
+                    if(h.resetfreedpages && old >= P)
+                    {
+                        // page locked by a concurrent reset
+                        if(me == 0u)
+                            backOff(page, k);
+                        if(formatted)
                         {
-                            if(oldcs >= bmax && oldcs <= maxcs)
-                                cs = oldcs;
-                            else
-                            {
-                                if(me == 0u)
-                                    atom::sub(h.pteCount(page), k);
-                                return 0u;
-                            }
+                            // D5 of scatter_heap.hpp, for the wave
+                            if(me == 0u)
+                                while(atom::load(h.pteCount(page)) >= P)
+                                    __builtin_amdgcn_s_sleep(1);
+                            __builtin_amdgcn_wave_barrier();
+                            continue;
+                        }
+                        return 0u;
+                    }
+                    bool snapValid = true;
+                    if(oldcs != 0u && oldcs != cs)
+                    {
+                        if(oldcs >= bmax && oldcs <= maxcs)
+                        {
+                            cs = oldcs; // formatted meanwhile with a fitting size
+                            L = layout(page, cs);
+                            snapValid = false;
+                        }
+                        else
+                        {
+                            if(me == 0u)
+                                backOff(page, k);
+                            return 0u;
                         }
                     }
                     const u32 cap = h.pageCapacity(cs);
@@ -187,48 +274,29 @@ namespace mallocMC
                     u32 got = 0u;
                     u64 served = 0u;
                     if(grant != 0u)
-                        served = claimBits(page, cs, grant, need, got, out);
+                        served = claimBits(page, cs, L, grant, need, got, out, snapValid, snap);
                     if(got < k && me == 0u)
-                        atom::sub(h.pteCount(page), k - got);
+                        backOff(page, k - got);
                     return served;
                 }
             }
 
             //! Claims `grant` free bits on the page's mask words and hands the
             //! chunks to the lanes of `need` in rank order.  Returns lanes served.
-            MALLOCMC_D u64 claimBits(u32 page, u32 cs, u32 grant, u64 need, u32& gotTotal, char*& out) const
+            //! Pass 0 chooses from `snap0` (lane w: word w) when `haveSnap`.
+            MALLOCMC_D u64 claimBits(u32 page, u32 cs, const Layout& L, u32 grant, u64 need, u32& gotTotal,
+                                     char*& out, bool haveSnap, u32 snap0) const
             {
                 const u32 me = hw::laneid();
                 const u32 P = h.pagesize;
-                const bool hier = cs <= h.hierarchyThreshold;
-                u32 nWords, full = 0u, additional = 0u;
-                u32* words;
-                if(hier)
-                {
-                    const u32 segmentsize = cs * 32u + 4u;
-                    full = umin32(32u, P / segmentsize);
-                    additional = h.calcAdditional(full, segmentsize, cs);
-                    nWords = full + (additional > 0u ? 1u : 0u);
-                    words = h.onPageMasks(page, nWords);
-                }
-                else
-                {
-                    nWords = 1u;
-                    words = h.pteBitmask(page);
-                }
+                const bool hier = L.hier;
+                const u32 nWords = L.nWords, full = L.full, additional = L.additional;
+                u32* const words = L.words;
                 const u32 myRank = hw::popc(need & hw::lanemask_lt());
                 const bool inNeed = ((need >> me) & 1u) != 0u;
                 u64 served = 0u;
                 u32 remaining = grant;
                 gotTotal = 0u;
-                // hint mask: segments known full (hierarchical pages only)
-                u32 hints = 0u;
-                if(hier)
-                {
-                    if(me == 0u)
-                        hints = atom::load(h.pteBitmask(page));
-                    hints = wave::lane(hints, 0u);
-                }
                 for(u32 pass = 0; pass < 3u && remaining > 0u; ++pass)
                 {
                     // lane w < nWords snapshots mask word w
@@ -237,8 +305,7 @@ namespace mallocMC
                     if(owner)
                     {
                         spots = hier ? (me < full ? 32u : additional) : umin32(P / cs, 32u);
-                        const bool skipHint = hier && pass == 0u && ((hints >> me) & 1u);
-                        snap = skipHint ? ~0u : atom::load(&words[me]);
+                        snap = (pass == 0u && haveSnap) ? snap0 : atom::load(&words[me]);
                     }
                     const u32 sel = spots == 32u ? ~0u : ((1u << spots) - 1u);
                     const u32 freeBits = owner ? (~snap & sel) : 0u;
@@ -294,7 +361,8 @@ namespace mallocMC
             //! hashed page, region by region, two passes with the 3/4 fill
             //! threshold first) but examines up to 64 pages -- 64/regionsize
             //! regions -- per round trip.
-            MALLOCMC_D u64 allocGroup(u64 group, u32 c, u32 bmax, u32 maxcs, char*& out) const
+            //! `ffb`: the access-block cursor (_firstfreeblock) as the wave read it.
+            MALLOCMC_D u64 allocGroup(u64 group, u32 c, u32 bmax, u32 maxcs, u32 ffb, char*& out) const
             {
                 const u32 me = hw::laneid();
                 const u32 R = h.regionsize;
@@ -310,10 +378,6 @@ namespace mallocMC
                 }
                 else
                     startInBlock = hashBase % ppb;
-                u32 ffb = 0u;
-                if(me == 0u)
-                    ffb = atom::load(&h.scalars->firstfreeblock);
-                ffb = wave::lane(ffb, 0u);
                 const u32 globalStart = startInBlock + ffb * ppb;
                 const u32 region0 = globalStart / R;
                 const u32 off0 = globalStart % R; // every region is entered at this page
@@ -381,25 +445,10 @@ namespace mallocMC
                                     used &= ~(1ull << l);
                                     cands &= ~(1ull << l);
                                     const u32 page = regionK * R + (R >= 64u ? base + l : l - lo);
-                                    u32 cs = wave::lane(pcs, l);
-                                    bool formatted = false;
-                                    if(cs == 0u)
-                                    {
-                                        u32 old = 0u;
-                                        if(me == 0u)
-                                            old = atom::cas(h.pteChunksize(page), 0u, c);
-                                        old = wave::lane(old, 0u);
-                                        if(old == 0u)
-                                        {
-                                            cs = c;
-                                            formatted = true;
-                                        }
-                                        else if(old >= bmax && old <= maxcs)
-                                            cs = old;
-                                        else
-                                            continue;
-                                    }
-                                    const u64 served = claimOnPage(page, cs, bmax, maxcs, need, formatted, out);
+                                    const u32 pc = wave::lane(pcs, l);
+                                    // a free page is formatted by the claim itself
+                                    const u64 served = claimOnPage(page, pc == 0u ? c : pc, bmax, maxcs, need,
+                                                                   pc == 0u, out);
                                     need &= ~served;
                                 }
                                 if(need == before && fillK + 1u <= R && me == 0u)
@@ -513,6 +562,13 @@ namespace mallocMC
                 u64 pending = hw::ballot(want);
                 const u32 cls = want ? sizeClass(h, b, t.classSplit) : 0xffffffffu;
                 const u32 minAlloc = umax32(b, h.paddedMinChunk);
+                // the access-block cursor, read once for all groups of the wave
+                // (the reference reads it per request; any recent value is a
+                // valid start, Scatter.hpp:557-560)
+                u32 ffb = 0u;
+                if(pending != 0u && me == 0u)
+                    ffb = atom::load(&h.scalars->firstfreeblock);
+                ffb = wave::lane(ffb, 0u);
                 while(pending != 0u)
                 {
                     const u32 leader = hw::ffs(pending) - 1u;
@@ -537,7 +593,7 @@ namespace mallocMC
                     const u32 bmin = wave::min(member ? b : 0xffffffffu);
                     const u32 maxcs = umin32(h.pagesize, umax32(h.wastefactor * bmin, h.paddedMinChunk));
                     char* mine = nullptr;
-                    const u64 served = allocGroup(group, c, bmax, maxcs, mine);
+                    const u64 served = allocGroup(group, c, bmax, maxcs, ffb, mine);
                     if(member && ((served >> me) & 1u))
                         result = mine;
                     // lanes of the group not served: the heap has no room for them
