@@ -52,6 +52,11 @@ namespace mallocMC
             kFixPagesizeLeak = 1u << 0,
         };
 
+        //! Upper bound on the sleeps a lane spends waiting for a concurrent
+        //! page reset to unlock (about 30 ms); past it the lane gives up on
+        //! that page instead of waiting forever on a count that never drops.
+        constexpr u32 kResetWaitLimit = 1u << 20;
+
         //! The three mutable heap scalars, each on its own 128-byte L2 line.
         struct alignas(128) ScatterScalars
         {
@@ -391,6 +396,7 @@ namespace mallocMC
             template<class Ids>
             MALLOCMC_D char* tryUsePage(u32 page, u32 chunksize, u32 bytes, u32 maxchunksize, bool formatted) const
             {
+                u32 waited = 0u;
                 for(;;)
                 {
                     char* chunk = nullptr;
@@ -428,10 +434,15 @@ namespace mallocMC
                     // would strand the page formatted but empty; the reset is a
                     // short straight-line section, so wait for it and use the page.
                     // Never taken on a serialized stream (count stays < pagesize).
-                    if(chunk == nullptr && formatted && resetfreedpages && filllevel >= pagesize)
+                    if(chunk == nullptr && formatted && resetfreedpages && filllevel >= pagesize
+                       && waited < kResetWaitLimit)
                     {
-                        while(atom::load(pteCount(page)) >= pagesize)
+                        while(atom::load(pteCount(page)) >= pagesize && waited < kResetWaitLimit)
+                        {
                             __builtin_amdgcn_s_sleep(1);
+                            ++waited;
+                        }
+                        ++waited;
                         continue;
                     }
                     return chunk;

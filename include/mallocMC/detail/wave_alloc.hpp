@@ -165,24 +165,34 @@ namespace mallocMC
                 return L;
             }
 
-            //! One lane: resets an empty page -- lock (count 0 -> pagesize),
-            //! clear its segment masks, drop its chunk size, unlock -- if no
-            //! one holds a reservation on it.  Orders the unlock after the
+            //! One lane: resets an empty chunked page -- lock (count 0 ->
+            //! pagesize), clear its segment masks, drop its chunk size, unlock
+            //! -- if no one holds a reservation on it.  Pages of a multi-page
+            //! allocation (chunk size above the page, count 0 while they are
+            //! in use) and pagesize chunks the reference releases through the
+            //! page-based path are left alone.  Orders the unlock after the
             //! other writes as the wave free does.
             MALLOCMC_D void resetEmpty(u32 page) const
             {
                 const u32 P = h.pagesize;
+                const u32 seen = atom::load(h.pteChunksize(page));
+                const bool chunked = seen != 0u && (seen < P || (seen == P && (h.flags & kFixPagesizeLeak)));
+                if(!chunked)
+                    return;
                 if(atom::cas(h.pteCount(page), 0u, P) != 0u)
                     return;
+                // under the lock the chunk size can only go from 0 to a value
+                // (a formatter or markpages); re-read it
                 const u32 cs = atom::load(h.pteChunksize(page));
+                const bool still = cs != 0u && (cs < P || (cs == P && (h.flags & kFixPagesizeLeak)));
                 u32 z = 0u;
-                if(cs != 0u && cs <= h.hierarchyThreshold)
+                if(still && cs <= h.hierarchyThreshold)
                 {
                     const Layout L = layout(page, cs);
                     for(u32 w = 0; w < L.nWords; ++w)
                         z |= atom::exch(&L.words[w], 0u);
                 }
-                const u32 oc = cs != 0u ? atom::cas(h.pteChunksize(page), cs, 0u) : 0u;
+                const u32 oc = still ? atom::cas(h.pteChunksize(page), cs, 0u) : 0u;
                 hw::consume(z ^ oc);
                 atom::sub(h.pteCount(page), P);
             }
@@ -219,6 +229,7 @@ namespace mallocMC
                 const u32 P = h.pagesize;
                 const bool recheck = h.resetfreedpages || fresh;
                 bool formatted = false;
+                u32 waited = 0u;
                 for(;;)
                 {
                     const u32 k = hw::popc(need);
@@ -242,13 +253,18 @@ namespace mallocMC
                         // page locked by a concurrent reset
                         if(me == 0u)
                             backOff(page, k);
-                        if(formatted)
+                        if(formatted && waited < kResetWaitLimit)
                         {
-                            // D5 of scatter_heap.hpp, for the wave
+                            // D5 of scatter_heap.hpp, for the wave (bounded:
+                            // after the limit the group moves on)
+                            u32 spins = waited;
                             if(me == 0u)
-                                while(atom::load(h.pteCount(page)) >= P)
+                                while(atom::load(h.pteCount(page)) >= P && spins < kResetWaitLimit)
+                                {
                                     __builtin_amdgcn_s_sleep(1);
-                            __builtin_amdgcn_wave_barrier();
+                                    ++spins;
+                                }
+                            waited = wave::lane(spins, 0u) + 1u;
                             continue;
                         }
                         return 0u;

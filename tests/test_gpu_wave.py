@@ -122,7 +122,7 @@ def test_wave_xmallocsimd_and_large(mmc, cuda):
     a.close()
 
 
-def _step_setup(torch, mmc, nheaps, slots_per_heap, heap_bytes, pagesize, seed=7):
+def _step_setup(torch, mmc, nheaps, slots_per_heap, heap_bytes, pagesize, seed=7, nsizes=8):
     cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=heap_bytes // 4, regionsize=16,
                             resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
     allocs = [mmc.ScatterAllocator(cfg, heap_bytes) for _ in range(nheaps)]
@@ -132,7 +132,7 @@ def _step_setup(torch, mmc, nheaps, slots_per_heap, heap_bytes, pagesize, seed=7
     for hidx in range(nheaps):
         base = hidx * slots_per_heap
         tiles += list(range(base, base + slots_per_heap, 64))
-    table = wl.ZIPF_SIZES[:8]
+    table = wl.ZIPF_SIZES[:nsizes]
     w = 1.0 / np.arange(1, len(table) + 1) ** 1.2
     cdf = np.minimum(np.round(np.cumsum(w / w.sum()) * 2 ** 32), 2 ** 32 - 1).astype(np.uint64)
     cdf[-1] = 2 ** 32 - 1
@@ -244,3 +244,28 @@ def test_multi_heap_steps_and_graph_replay(mmc, cuda):
     multi.close()
     for a in allocs:
         a.close()
+
+
+def test_multi_heap_steps_with_multipage_requests(mmc, cuda):
+    """Pagesize 4 KiB with sizes up to 64 KiB: multi-page runs (markpages) and
+    chunk claims race on the same free pages in one launch.  A claim that
+    loses a free page to a run must leave the run's pages alone; live
+    allocations never overlap and free-all restores every heap."""
+    torch = cuda
+    nheaps, per = 4, 8192
+    allocs, multi, dev, d, slot_heap = _step_setup(torch, mmc, nheaps, per, 128 * MiB, 4096, seed=11, nsizes=13)
+    s0 = [[a.getAvailableSlots(s) for s in (64, 4096, 8 * 4096)] for a in allocs]
+    for agg in (1, 64):
+        tun = mmc.Tuning(persistent=True, aggregation=agg)
+        print(f"multipage steps aggregation={agg}", flush=True)
+        for _ in range(20):
+            multi.step(d, tun)
+        torch.cuda.synchronize()
+        _check_live(torch, allocs, dev, slot_heap)
+    c = dev["counters"].cpu().numpy()
+    assert c[0] > 0 and c[1] > 0
+    for hidx, a in enumerate(allocs):
+        sel = torch.from_numpy(np.nonzero(slot_heap == hidx)[0]).cuda()
+        a.wave_free_batch(dev["offs"][sel].contiguous())
+    torch.cuda.synchronize()
+    assert [[a.getAvailableSlots(s) for s in (64, 4096, 8 * 4096)] for a in allocs] == s0
