@@ -1,6 +1,8 @@
 """GPU test of the fragmentation / OOM stress cycle (config 4 shape on a
 1 GiB heap): fill until null, free a random half, refill; invariants after
-every fill and the slot counts restored at the end."""
+every fill and the slot counts restored at the end.  The fill reached at the
+first null is compared with the reference protocol's (the oracle, serialized)
+on the same heap."""
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -18,3 +20,19 @@ def test_oom_stress_cycles(mmc, cuda, split, first_null):
     assert r["slots_restored"], r
     for c in r["cycles"]:
         assert c["fill_fraction"] > 0.5, c
+
+
+def test_oom_first_null_fill_close_to_reference_protocol(mmc, cuda, oracle_mod):
+    """65,536 concurrent lanes on a 256 MiB heap reach, at their first null,
+    a payload fill within 0.08 of what the serialized reference protocol
+    reaches with the same size mix, and no page is left free."""
+    from mallocmc_amd import oomstress
+    from oom_fill import oracle_first_null_fill
+    heap = 256 * 1024 ** 2
+    ref = oracle_first_null_fill(oracle_mod, heap, cycles=3)
+    r = oomstress.run_config4(heap_bytes=heap, lanes=65536, cycles=3, log=print)
+    assert r["invariants_ok"] and r["slots_restored"], r
+    for c, o in zip(r["cycles"], ref):
+        print(f"fill gpu {c['fill_fraction']:.3f} reference protocol {o['fill_fraction']:.3f}", flush=True)
+        assert c["slots_after_fill"][3] == 0, c  # no free page left at the first null
+        assert c["fill_fraction"] > o["fill_fraction"] - 0.08, (c, o)

@@ -278,3 +278,16 @@ def test_concurrent_baseline_invariants(oracle_mod):
                 assert all(h.pte(p)[1] == 0 for p in range(h.geometry.numpages))
                 if restore:
                     assert h.available_slots(16) == s0
+
+
+def test_first_null_fill_of_reference_protocol(oracle_mod):
+    """Config-4 size mix on a 128 MiB heap, pagesize 64 KiB: the reference's
+    protocol returns its first null only once no page is free (null only when
+    no page can take the request), and its payload fill at that point stays
+    well below 95%: a request above half a page takes a whole page (one
+    chunk per page, Scatter.hpp:494-510)."""
+    from oom_fill import oracle_first_null_fill
+    res = oracle_first_null_fill(oracle_mod, 128 * 1024 ** 2)
+    for r in res:
+        assert r["free_pages"] == 0, r
+        assert 0.55 < r["fill_fraction"] < 0.8, r
