@@ -48,33 +48,75 @@ namespace mallocMC
             //! Size classes per octave = 2^classSplit (0..3): finer classes waste
             //! less (chunk <= (1 + 2^-classSplit) x request) but aggregate fewer lanes.
             u32 classSplit = 0u;
+            //! Claim chunks above the hierarchy threshold on several pages in
+            //! one round trip (claimFlat).
+            u32 flatFast = 1u;
         };
 
+        //! Wave-wide reductions and scans.  All 64 lanes must be active.
+        //! Reductions use DPP (data-parallel lane moves inside the VALU: quad
+        //! butterflies, row rotates and the GFX9 row broadcasts), not LDS
+        //! permutes, so a reduction costs a few VALU cycles instead of six
+        //! LDS round trips.
         namespace wave
         {
+            //! DPP controls (GFX9): quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+            //! row_ror:4, row_ror:8, row_bcast:15, row_bcast:31.
+            constexpr int kQuadXor1 = 0xB1;
+            constexpr int kQuadXor2 = 0x4E;
+            constexpr int kRowRor4 = 0x124;
+            constexpr int kRowRor8 = 0x128;
+            constexpr int kRowBcast15 = 0x142;
+            constexpr int kRowBcast31 = 0x143;
+
+            template<int Ctrl, int RowMask = 0xf>
+            MALLOCMC_D u32 dpp(u32 v)
+            {
+                // lanes outside RowMask keep v itself
+                return static_cast<u32>(__builtin_amdgcn_update_dpp(
+                    static_cast<int>(v),
+                    static_cast<int>(v),
+                    Ctrl,
+                    RowMask,
+                    0xf,
+                    false));
+            }
+
+            //! Combines all 64 lanes with an associative, commutative op;
+            //! lane 63 ends with the result, which is returned to all lanes.
+            template<class Op>
+            MALLOCMC_D u32 reduce(u32 v, Op op)
+            {
+                v = op(v, dpp<kQuadXor1>(v));
+                v = op(v, dpp<kQuadXor2>(v)); // quad value in every lane
+                v = op(v, dpp<kRowRor4>(v));
+                v = op(v, dpp<kRowRor8>(v)); // row (16 lanes) value in every lane
+                v = op(v, dpp<kRowBcast15, 0xa>(v)); // rows 1, 3 add rows 0, 2
+                v = op(v, dpp<kRowBcast31, 0xc>(v)); // rows 2, 3 add lane 31
+                return static_cast<u32>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+            }
+
             MALLOCMC_D u32 max(u32 v)
             {
-                for(int d = 32; d > 0; d >>= 1)
-                {
-                    const u32 o = __shfl_xor(v, d, 64);
-                    v = v > o ? v : o;
-                }
-                return v;
+                return reduce(v, [](u32 a, u32 b) { return a > b ? a : b; });
             }
             MALLOCMC_D u32 min(u32 v)
             {
-                for(int d = 32; d > 0; d >>= 1)
-                {
-                    const u32 o = __shfl_xor(v, d, 64);
-                    v = v < o ? v : o;
-                }
-                return v;
+                return ~reduce(~v, [](u32 a, u32 b) { return a > b ? a : b; });
             }
             MALLOCMC_D u32 bor(u32 v)
             {
-                for(int d = 32; d > 0; d >>= 1)
-                    v |= __shfl_xor(v, d, 64);
-                return v;
+                return reduce(v, [](u32 a, u32 b) { return a | b; });
+            }
+            //! Exclusive prefix sum over lanes of values below 64, bit by bit
+            //! with ballots (scalar ops only).
+            MALLOCMC_D u32 prefix6(u32 v)
+            {
+                const u64 lt = hw::lanemask_lt();
+                u32 p = 0u;
+                for(u32 b = 0; b < 6u; ++b)
+                    p += static_cast<u32>(hw::popc(hw::ballot(((v >> b) & 1u) != 0u) & lt)) << b;
+                return p;
             }
             //! Value of lane `l` (l uniform).
             MALLOCMC_D u32 lane(u32 v, u32 l)
@@ -292,7 +334,12 @@ namespace mallocMC
                     if(grant != 0u)
                         served = claimBits(page, cs, L, grant, need, got, out, snapValid, snap);
                     if(got < k && me == 0u)
-                        backOff(page, k - got);
+                    {
+                        if(got == 0u)
+                            backOff(page, k);
+                        else
+                            atom::sub(h.pteCount(page), k - got); // holds got chunks: count stays > 0
+                    }
                     return served;
                 }
             }
@@ -326,18 +373,8 @@ namespace mallocMC
                     const u32 sel = spots == 32u ? ~0u : ((1u << spots) - 1u);
                     const u32 freeBits = owner ? (~snap & sel) : 0u;
                     const u32 freeCnt = hw::popc(freeBits);
-                    // exclusive prefix of free counts over words 0..me-1
-                    u32 prefix = 0u;
-                    {
-                        u32 v = freeCnt;
-                        for(int d = 1; d < 64; d <<= 1)
-                        {
-                            const u32 o = __shfl_up(v, static_cast<unsigned>(d), 64);
-                            if(me >= static_cast<u32>(d))
-                                v += o;
-                        }
-                        prefix = v - freeCnt;
-                    }
+                    // exclusive prefix of free counts (<= 32) over words 0..me-1
+                    const u32 prefix = wave::prefix6(freeCnt);
                     u32 take = 0u;
                     if(owner && prefix < remaining)
                         take = umin32(freeCnt, remaining - prefix);
@@ -372,6 +409,117 @@ namespace mallocMC
                 return served;
             }
 
+            //! Claims chunks for the lanes of `need` on several flat pages of one
+            //! region at once.  Lane l stands for page `pageL` of the scan
+            //! (chunk size pcs, count pcount, PTE bitmask pbits as the scan saw
+            //! them); `used` / `fresh` are the candidate lanes.  Pages are
+            //! taken in the scan's preference order (used pages from the
+            //! rotation point, then fresh pages) until their free chunks cover
+            //! the group.  Each chosen page's lane issues, in one round trip,
+            //! the count reservation, the chunk-size check (which formats a
+            //! fresh page) and an atomicOr of the chunk bits it picked from the
+            //! scanned bitmask: ownership of a chunk is decided by the
+            //! atomicOr's result as in claimBits, so picking from a stale
+            //! bitmask only costs chunks, never hands one out twice.  A page
+            //! that turned out locked by a reset or formatted with another size
+            //! gives back what it won.  Returns the lanes served.
+            MALLOCMC_D u64 claimFlat(u32 pageL, u32 pcs, u32 pcount, u32 pbits, u64 used, u64 fresh, u32 rot,
+                                     u32 c, u64 need, u64& chosen, char*& out) const
+            {
+                const u32 me = hw::laneid();
+                const u32 P = h.pagesize;
+                const bool isUsed = ((used >> me) & 1u) != 0u;
+                const bool isFresh = ((fresh >> me) & 1u) != 0u;
+                const u32 csl = isUsed ? pcs : c;
+                const u32 capl = (isUsed || isFresh) ? umin32(P / csl, 32u) : 0u;
+                const u32 selBits = capl >= 32u ? ~0u : ((1u << capl) - 1u);
+                const u32 freeBits = (isUsed || isFresh) ? (~pbits & selBits) : 0u;
+                const u32 room = isUsed ? (capl > pcount ? capl - pcount : 0u) : capl;
+                const u32 f = umin32(static_cast<u32>(hw::popc(freeBits)), room);
+                // position of this page in the preference order
+                const u64 hiMask = rot < 64u ? (~0ull << rot) : 0ull;
+                const bool inUhi = isUsed && ((hiMask >> me) & 1u);
+                const bool inUlo = isUsed && !inUhi;
+                const bool inFhi = isFresh && ((hiMask >> me) & 1u);
+                const bool inFlo = isFresh && !inFhi;
+                const u32 pUhi = wave::prefix6(inUhi ? f : 0u);
+                const u32 pUlo = wave::prefix6(inUlo ? f : 0u);
+                const u32 pFhi = wave::prefix6(inFhi ? f : 0u);
+                const u32 pFlo = wave::prefix6(inFlo ? f : 0u);
+                const u32 tUhi = wave::lane(pUhi + (inUhi ? f : 0u), 63u);
+                const u32 tUlo = wave::lane(pUlo + (inUlo ? f : 0u), 63u);
+                const u32 tFhi = wave::lane(pFhi + (inFhi ? f : 0u), 63u);
+                u32 pos = 0u;
+                if(inUhi)
+                    pos = pUhi;
+                else if(inUlo)
+                    pos = tUhi + pUlo;
+                else if(inFhi)
+                    pos = tUhi + tUlo + pFhi;
+                else if(inFlo)
+                    pos = tUhi + tUlo + tFhi + pFlo;
+                const u32 k = static_cast<u32>(hw::popc(need));
+                const u32 take = (f != 0u && pos < k) ? umin32(f, k - pos) : 0u;
+                const u32 choose = take != 0u ? wave::lowestBits(freeBits, take) : 0u;
+                chosen = hw::ballot(take != 0u);
+                // one round trip for every chosen page
+                u32 old = 0u, oldcs = 0u, oldbits = 0u;
+                if(take != 0u)
+                {
+                    old = atom::add(h.pteCount(pageL), take);
+                    oldcs = atom::cas(h.pteChunksize(pageL), 0u, csl);
+                    oldbits = atom::or_(h.pteBitmask(pageL), choose);
+                }
+                const bool formatted = take != 0u && oldcs == 0u;
+                const bool csOk = formatted || oldcs == csl;
+                const bool locked = h.resetfreedpages && old >= P;
+                const u32 won = take != 0u ? (choose & ~oldbits) : 0u;
+                const u32 keep = (take != 0u && csOk && !locked) ? won : 0u;
+                const u32 kept = static_cast<u32>(hw::popc(keep));
+                if(take != 0u)
+                {
+                    if(won & ~keep)
+                        atom::and_(h.pteBitmask(pageL), ~(won & ~keep));
+                    if(kept == 0u)
+                    {
+                        backOff(pageL, take);
+                        if(formatted && locked)
+                        {
+                            // this lane formatted a page whose reset still held
+                            // the count lock: once the reset is done, take the
+                            // chunk size back unless someone uses the page (D5)
+                            u32 spins = 0u;
+                            while(atom::load(h.pteCount(pageL)) >= P && spins < kResetWaitLimit)
+                            {
+                                __builtin_amdgcn_s_sleep(1);
+                                ++spins;
+                            }
+                            resetEmpty(pageL);
+                        }
+                    }
+                    else if(kept < take)
+                        atom::sub(h.pteCount(pageL), take - kept); // holds kept chunks: count stays > 0
+                }
+                // hand the kept chunks to the group's lanes in rank order
+                const u32 myRank = static_cast<u32>(hw::popc(need & hw::lanemask_lt()));
+                const bool inNeed = ((need >> me) & 1u) != 0u;
+                u32 got = 0u;
+                for(u64 pagesKept = hw::ballot(kept != 0u); pagesKept != 0u; pagesKept &= pagesKept - 1u)
+                {
+                    const u32 l = hw::ffs(pagesKept) - 1u;
+                    const u32 bits = wave::lane(keep, l);
+                    const u32 n = static_cast<u32>(hw::popc(bits));
+                    if(inNeed && myRank >= got && myRank < got + n)
+                    {
+                        const u32 pg = wave::lane(pageL, l);
+                        const u32 cs = wave::lane(csl, l);
+                        out = h.pageData(pg) + static_cast<u64>(wave::selectBit(bits, myRank - got)) * cs;
+                    }
+                    got += n;
+                }
+                return hw::ballot(inNeed && myRank < got);
+            }
+
             //! Group allocation of chunk size c for the lanes in `group`.  Walks
             //! the regions in the reference's order (Scatter.hpp:563-635: from the
             //! hashed page, region by region, two passes with the 3/4 fill
@@ -402,6 +550,10 @@ namespace mallocMC
                 const u32 span = R >= 64u ? 64u : W * R;
                 u64 need = group;
                 u32 checklevel = R * 3u / 4u;
+                // chunks above the hierarchy threshold: every fitting page keeps
+                // its chunk bits in the PTE bitmask (candidates have a chunk
+                // size >= bmax = c)
+                const bool flat = c > h.hierarchyThreshold && t.flatFast != 0u;
                 for(u32 finder = 0; finder < 2u && need != 0u; ++finder)
                 {
                     for(u32 done = 0; done < numregions && need != 0u; done += W)
@@ -410,7 +562,7 @@ namespace mallocMC
                         // lane -> (region of this round, page within region)
                         const u32 ri = R >= 64u ? 0u : me / R;
                         const u32 region = (region0 + done + ri) % numregions;
-                        u32 fill = 0xffffffffu, pcs = 0u, pcount = 0u;
+                        u32 fill = 0xffffffffu, pcs = 0u, pcount = 0u, pbits = 0u;
                         const bool inSpan = me < span && ri < nreg;
                         for(u32 base = 0; base < (R >= 64u ? R : 1u) && need != 0u; base += 64u)
                         {
@@ -421,6 +573,8 @@ namespace mallocMC
                                 fill = atom::load(&h.regions[region]);
                                 pcs = atom::load(h.pteChunksize(region * R + pj));
                                 pcount = atom::load(h.pteCount(region * R + pj));
+                                if(flat)
+                                    pbits = atom::load(h.pteBitmask(region * R + pj));
                             }
                             const bool regionOpen = valid && fill < checklevel;
                             bool cand = false;
@@ -449,6 +603,20 @@ namespace mallocMC
                                 const u32 rotAbs = R >= 64u ? (off0 >= base ? off0 - base : 0u) : off0;
                                 const u32 rot = lo + rotAbs;
                                 const u64 before = need;
+                                if(flat)
+                                {
+                                    // two multi-page rounds over the scanned candidates; pages a
+                                    // round chose are not offered again
+                                    const u32 pageL = regionK * R + (R >= 64u ? base + me : me - lo);
+                                    for(u32 fr = 0; fr < 2u && need != 0u && (used | cands) != 0u; ++fr)
+                                    {
+                                        u64 chosen = 0u;
+                                        need &= ~claimFlat(pageL, pcs, pcount, pbits, used, cands, rot, c, need, chosen,
+                                                           out);
+                                        used &= ~chosen;
+                                        cands &= ~chosen;
+                                    }
+                                }
                                 // partially used pages of a fitting chunk size first, free
                                 // pages after them: formatting a fresh page while a fitting
                                 // one has room is what fragments the reference's heap
@@ -561,6 +729,13 @@ namespace mallocMC
             MALLOCMC_D void* malloc(u32 bytesIn) const
             {
                 const u32 me = hw::laneid();
+                // the access-block cursor, read once for all groups of the wave
+                // (the reference reads it per request; any recent value is a
+                // valid start, Scatter.hpp:557-560); issued first so that its
+                // round trip overlaps the size loads
+                u32 ffb = 0u;
+                if(me == 0u)
+                    ffb = atom::load(&h.scalars->firstfreeblock);
                 Dist dist;
                 const u32 b = dist.collect(h, h.pad(bytesIn));
                 char* result = nullptr;
@@ -578,12 +753,6 @@ namespace mallocMC
                 u64 pending = hw::ballot(want);
                 const u32 cls = want ? sizeClass(h, b, t.classSplit) : 0xffffffffu;
                 const u32 minAlloc = umax32(b, h.paddedMinChunk);
-                // the access-block cursor, read once for all groups of the wave
-                // (the reference reads it per request; any recent value is a
-                // valid start, Scatter.hpp:557-560)
-                u32 ffb = 0u;
-                if(pending != 0u && me == 0u)
-                    ffb = atom::load(&h.scalars->firstfreeblock);
                 ffb = wave::lane(ffb, 0u);
                 while(pending != 0u)
                 {

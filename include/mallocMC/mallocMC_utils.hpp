@@ -88,10 +88,13 @@ namespace mallocMC
             return __shfl(v, static_cast<int>(lane), 64);
         }
 
+        //! 64-bit value of lane `lane`; `lane` must be the same in all lanes.
         MALLOCMC_D std::uint64_t readlane64(std::uint64_t v, std::uint32_t lane)
         {
-            const std::uint32_t lo = __shfl(static_cast<std::uint32_t>(v), static_cast<int>(lane), 64);
-            const std::uint32_t hi = __shfl(static_cast<std::uint32_t>(v >> 32), static_cast<int>(lane), 64);
+            const std::uint32_t lo = static_cast<std::uint32_t>(
+                __builtin_amdgcn_readlane(static_cast<int>(static_cast<std::uint32_t>(v)), static_cast<int>(lane)));
+            const std::uint32_t hi = static_cast<std::uint32_t>(
+                __builtin_amdgcn_readlane(static_cast<int>(static_cast<std::uint32_t>(v >> 32)), static_cast<int>(lane)));
             return (static_cast<std::uint64_t>(hi) << 32) | lo;
         }
 

@@ -24,6 +24,8 @@ typedef struct mmc_tuning {
     uint32_t persistent;   /* 0: one lane per request; 1: persistent waves pull 64-request tiles */
     uint32_t waves_per_cu; /* persistent launch size (0 = 8) */
     uint32_t class_split;  /* size classes per octave = 2^class_split (0..3) */
+    uint32_t flat_single;  /* 1: claim pages of chunks above the hierarchy threshold one at a time
+                              (0, the default: several pages per round trip) */
 } mmc_tuning;
 
 int mmc_default_tuning(mmc_tuning* t);

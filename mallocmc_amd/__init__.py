@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = [
     "mmc_info", "mmc_available_slots", "mmc_available_slots_async", "mmc_get_handle",
     "mmc_malloc_batch", "mmc_free_batch", "mmc_serial_stream",
     "mmc_write_pattern", "mmc_verify_pattern", "mmc_alloc_until_null", "mmc_bench_atomics",
-    "mmc_bench_copy",
+    "mmc_bench_copy", "mmc_selftest_wave",
 ]
 BATCHED_SYMBOLS = [
     "mmc_default_tuning", "mmc_wave_malloc_batch", "mmc_wave_free_batch", "mmc_multi_create",
@@ -52,22 +52,25 @@ class MmcTuning(ctypes.Structure):
         ("persistent", ctypes.c_uint32),
         ("waves_per_cu", ctypes.c_uint32),
         ("class_split", ctypes.c_uint32),
+        ("flat_single", ctypes.c_uint32),
     ]
 
 
 @dataclasses.dataclass
 class Tuning:
     """Wave-path tuning (mmc_tuning): aggregation width 1..64, XCD slices,
-    persistent launch and its waves per CU, size classes per octave."""
+    persistent launch and its waves per CU, size classes per octave, and
+    whether flat pages are claimed one at a time (flat_single)."""
     aggregation: int = 64
     xcd_slices: int = 0
     persistent: bool = False
     waves_per_cu: int = 8
     class_split: int = 0
+    flat_single: bool = False
 
     def to_c(self) -> MmcTuning:
         return MmcTuning(self.aggregation, self.xcd_slices, int(bool(self.persistent)), self.waves_per_cu,
-                         self.class_split)
+                         self.class_split, int(bool(self.flat_single)))
 
 
 class MmcStepDesc(ctypes.Structure):
@@ -211,6 +214,7 @@ def lib() -> ctypes.CDLL:
         "mmc_alloc_until_null": ([vp, u32, vp, u64, vp, u32, u32, vp], i32),
         "mmc_bench_atomics": ([vp, u64, u32, u32, u32, u32, vp, vp], i32),
         "mmc_bench_copy": ([vp, vp, u64, vp], i32),
+        "mmc_selftest_wave": ([u32, u64, vp, vp], i32),
         "mmc_default_tuning": ([P(MmcTuning)], i32),
         "mmc_wave_malloc_batch": ([vp, vp, vp, u64, P(MmcTuning), vp], i32),
         "mmc_wave_free_batch": ([vp, vp, u64, P(MmcTuning), vp], i32),

@@ -30,6 +30,7 @@ namespace
             w.aggregation = t->aggregation == 0u ? 64u : (t->aggregation > 64u ? 64u : t->aggregation);
             w.xcdSlices = t->xcd_slices;
             w.classSplit = t->class_split > 3u ? 3u : t->class_split;
+            w.flatFast = t->flat_single ? 0u : 1u;
         }
         return w;
     }
@@ -304,6 +305,7 @@ int mmc_default_tuning(mmc_tuning* t)
     t->persistent = 0u;
     t->waves_per_cu = 8u;
     t->class_split = 0u;
+    t->flat_single = 0u;
     return MMC_OK;
 }
 

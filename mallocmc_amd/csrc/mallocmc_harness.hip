@@ -4,6 +4,7 @@
 #include "mallocmc_harness.h"
 
 #include "mallocMC/detail/device_allocator.hpp"
+#include "mallocMC/detail/wave_alloc.hpp"
 #include "mmc_internal.hpp"
 
 using namespace mallocMC;
@@ -142,6 +143,49 @@ namespace
     }
 } // namespace
 
+namespace
+{
+    //! Seeded values per lane; the wave's primitives against lane-by-lane loops.
+    __global__ __launch_bounds__(64) void waveSelfTestKernel(u64 seed, u32* result)
+    {
+        using namespace mallocMC::detail;
+        const u32 me = hw::laneid();
+        u64 z = seed + 0x9E3779B97F4A7C15ull * (static_cast<u64>(blockIdx.x) * 64u + me + 1u);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        // vary the value range by wave so that ties, zeros and all-ones occur
+        const u32 shift = blockIdx.x % 33u;
+        const u32 v = static_cast<u32>(z) >> shift;
+        const u32 small = static_cast<u32>(z >> 40) % 33u; // 0..32, the prefix domain
+        u32 bmax = 0u, bmin = 0xffffffffu, bor = 0u, bpre = 0u;
+        for(u32 l = 0; l < 64u; ++l)
+        {
+            const u32 o = static_cast<u32>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(l)));
+            const u32 so = static_cast<u32>(__builtin_amdgcn_readlane(static_cast<int>(small), static_cast<int>(l)));
+            bmax = o > bmax ? o : bmax;
+            bmin = o < bmin ? o : bmin;
+            bor |= o;
+            bpre += l < me ? so : 0u;
+        }
+        const u32 m0 = hw::ballot(wave::max(v) != bmax) != 0u ? 1u : 0u;
+        const u32 m1 = hw::ballot(wave::min(v) != bmin) != 0u ? 1u : 0u;
+        const u32 m2 = hw::ballot(wave::bor(v) != bor) != 0u ? 1u : 0u;
+        const u32 m3 = static_cast<u32>(hw::popc(hw::ballot(wave::prefix6(small) != bpre)));
+        if(me == 0u)
+        {
+            if(m0)
+                atom::add(&result[0], m0);
+            if(m1)
+                atom::add(&result[1], m1);
+            if(m2)
+                atom::add(&result[2], m2);
+            if(m3)
+                atom::add(&result[3], m3);
+        }
+    }
+} // namespace
+
 extern "C" {
 
 int mmc_write_pattern(mmc_heap* heap, const uint64_t* d_offsets, const uint32_t* d_sizes, uint64_t n, uint64_t salt,
@@ -204,6 +248,19 @@ int mmc_bench_atomics(uint32_t* d_buf, uint64_t words, uint32_t pattern, uint32_
                        pattern, iters, d_sink);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "atomics launch");
+}
+
+int mmc_selftest_wave(uint32_t waves, uint64_t seed, uint32_t* d_result, void* stream)
+{
+    if(!d_result || waves == 0u)
+        return mmc::fail(MMC_EINVAL, "bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemsetAsync(d_result, 0, 4u * sizeof(uint32_t), st);
+    if(err != hipSuccess)
+        return mmc::hipFail(err, "hipMemsetAsync(selftest)");
+    hipLaunchKernelGGL(waveSelfTestKernel, dim3(waves), dim3(64), 0, st, seed, d_result);
+    err = hipGetLastError();
+    return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "selftest launch");
 }
 
 int mmc_bench_copy(const void* d_src, void* d_dst, uint64_t bytes, void* stream)

@@ -269,3 +269,16 @@ def test_multi_heap_steps_with_multipage_requests(mmc, cuda):
         a.wave_free_batch(dev["offs"][sel].contiguous())
     torch.cuda.synchronize()
     assert [[a.getAvailableSlots(s) for s in (64, 4096, 8 * 4096)] for a in allocs] == s0
+
+
+def test_wave_primitives_selftest(mmc, cuda):
+    """The DPP reductions (max, min, or) and the ballot prefix sum agree with
+    lane-by-lane loops on 4096 seeded waves, value ranges 32..0 bits wide."""
+    import ctypes
+    torch = cuda
+    res = torch.zeros(4, dtype=torch.int32, device="cuda")
+    rc = mmc.lib().mmc_selftest_wave(4096, 12345, ctypes.c_void_p(res.data_ptr()),
+                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert res.cpu().tolist() == [0, 0, 0, 0]
