@@ -509,10 +509,10 @@ namespace mallocMC
             }
 
             //! Page::init (Scatter.hpp:249-258): clear the on-page mask area.
-            //! Clears a page's segment masks (the reference's page reset zeroes
-            //! the on-page masks, Scatter.hpp:691-697, 879-881).  Returning
-            //! exchanges: the caller may consume the result to order later
-            //! atomics after these have been performed.
+            //! Clears a page's segment masks (Page::init, Scatter.hpp:249-258;
+            //! the reference calls it when a page-based allocation is freed,
+            //! :879-881).  Returning exchanges: the caller may consume the
+            //! result to order later atomics after these have been performed.
             MALLOCMC_D u32 pageInit(u32 page) const
             {
                 u32* w = masks + static_cast<u64>(page) * maskStride;
@@ -552,10 +552,17 @@ namespace mallocMC
                 // whose decrement empties the page takes the lock with
                 // CAS(count, 0 -> P).  On a serialized stream both forms make the
                 // same state transitions and return the same old fill level.
+                // The reference also clears the page's on-page masks here
+                // (Page::init, Scatter.hpp:249-258, 691): they share the page
+                // with payload, which a flat chunk size may have overwritten.
+                // The side masks hold no payload, and every claimed bit is
+                // cleared by its free before the count drops, so they are
+                // already clear when the count reaches zero; clearing them
+                // again could wipe a bit a concurrent wave-path claim has just
+                // set ahead of its count reservation.
                 const u32 oldfill = atom::sub(pteCount(page), 1u);
                 if(resetfreedpages && oldfill == 1u && atom::cas(pteCount(page), 0u, P) == 0u)
                 {
-                    pageInit(page);
                     atom::cas(pteChunksize(page), chunksize, 0u);
                     atom::fence_device();
                     atom::sub(pteCount(page), P);

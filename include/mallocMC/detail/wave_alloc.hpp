@@ -208,7 +208,7 @@ namespace mallocMC
             }
 
             //! One lane: resets an empty chunked page -- lock (count 0 ->
-            //! pagesize), clear its segment masks, drop its chunk size, unlock
+            //! pagesize), drop its chunk size, unlock
             //! -- if no one holds a reservation on it.  Pages of a multi-page
             //! allocation (chunk size above the page, count 0 while they are
             //! in use) and pagesize chunks the reference releases through the
@@ -224,18 +224,12 @@ namespace mallocMC
                 if(atom::cas(h.pteCount(page), 0u, P) != 0u)
                     return;
                 // under the lock the chunk size can only go from 0 to a value
-                // (a formatter or markpages); re-read it
+                // (a formatter or markpages); re-read it.  No chunk bit is set
+                // at count zero, so no mask needs clearing.
                 const u32 cs = atom::load(h.pteChunksize(page));
                 const bool still = cs != 0u && (cs < P || (cs == P && (h.flags & kFixPagesizeLeak)));
-                u32 z = 0u;
-                if(still && cs <= h.hierarchyThreshold)
-                {
-                    const Layout L = layout(page, cs);
-                    for(u32 w = 0; w < L.nWords; ++w)
-                        z |= atom::exch(&L.words[w], 0u);
-                }
                 const u32 oc = still ? atom::cas(h.pteChunksize(page), cs, 0u) : 0u;
-                hw::consume(z ^ oc);
+                hw::consume(oc);
                 atom::sub(h.pteCount(page), P);
             }
 
@@ -894,33 +888,16 @@ namespace mallocMC
                 const u64 lockedLanes = hw::ballot(locked);
                 if(lockedLanes == 0u)
                     return;
-                // order as Scatter.hpp:691-710: clear masks, drop the chunk size,
-                // unlock (the count lock keeps allocators out meanwhile).  The
-                // reference fences the device before the unlock so that the
-                // mask stores are seen first.  Here every write of the reset is
-                // a returning atomic on metadata that only atomics touch (the
-                // side mask array and the PTE), so waiting for their results
-                // before the unlock is enough; no L2 write-back is needed.
-                // Only a hierarchical page has segment masks; the wave clears
-                // each locked page's words with one vector exchange (lane s
-                // takes word s of the page's 128-byte line).
-                u32 nseg = 0u;
-                if(locked && hier)
-                {
-                    const u32 segmentsize = cs * 32u + 4u;
-                    const u32 full = umin32(32u, P / segmentsize);
-                    nseg = full + (h.calcAdditional(full, segmentsize, cs) > 0u ? 1u : 0u);
-                }
-                u32 z = 0u;
-                for(u64 m = lockedLanes; m != 0u; m &= m - 1u)
-                {
-                    const u32 l = hw::ffs(m) - 1u;
-                    const u32 lpage = wave::lane(page, l);
-                    if(me < wave::lane(nseg, l))
-                        z |= atom::exch(h.onPageMasks(lpage, 0u) + me, 0u);
-                }
+                // order as Scatter.hpp:686-716: lock, drop the chunk size,
+                // unlock.  The reference also clears the masks (Page::init) and
+                // fences the device before the unlock.  The side masks are
+                // already clear when the count reaches zero (every free clears
+                // its bit before its count decrement), so nothing is cleared;
+                // the chunk-size CAS is a returning atomic on the PTE, and the
+                // unlock is issued once its result is back, so no L2
+                // write-back is needed.
                 const u32 oc = locked ? atom::cas(h.pteChunksize(page), cs, 0u) : 0u;
-                hw::consume(z ^ oc);
+                hw::consume(oc);
                 if(locked)
                     atom::sub(h.pteCount(page), P);
             }
