@@ -267,6 +267,13 @@ namespace mallocMC
             u32 accessblocks = 0;
             u32 pagesperblock = 0;
             u32 flags = 0;
+            //! divisions by heap constants: regionsize, numregions, pages per
+            //! access block, pagesize
+            FastDiv divR;
+            FastDiv divRegions;
+            FastDiv divPpb;
+            FastDiv divP;
+            FastDiv divSlice8; //!< pagesperblock / 8: one XCD's slice of an access block
 
             // ------------------------------------------------ addressing
             MALLOCMC_D u32* pteChunksize(u32 p) const
@@ -334,15 +341,23 @@ namespace mallocMC
                 }
             }
 
-            //! calcAdditionalChunks (Scatter.hpp:365-383), signed reading.
+            //! n / d for n <= pagesize: the float-reciprocal division where it is
+            //! exact for the quotients used here (all clamped to 32 or less).
+            MALLOCMC_D u32 pdiv(u32 n, u32 d) const
+            {
+                return pagesize < (1u << 24) ? udiv24(n, d) : n / d;
+            }
+
+            //! calcAdditionalChunks (Scatter.hpp:365-383), signed reading:
+            //! max(0, pagesize - full * segmentsize - 4) / chunksize, at most 31.
             MALLOCMC_D u32 calcAdditional(u32 fullsegments, u32 segmentsize, u32 chunksize) const
             {
                 if(fullsegments == 32u)
                     return 0u;
-                const long long rem = static_cast<long long>(pagesize)
-                    - static_cast<long long>(fullsegments) * static_cast<long long>(segmentsize) - 4ll;
-                const long long add = (rem < 0 ? 0ll : rem) / static_cast<long long>(chunksize);
-                return static_cast<u32>(add < 31 ? add : 31);
+                const u32 used = fullsegments * segmentsize + 4u; // <= pagesize + 4
+                if(used >= pagesize)
+                    return 0u;
+                return umin32(pdiv(pagesize - used, chunksize), 31u);
             }
 
             //! Number of chunks a page of the given chunk size holds (tryUsePage :494-510).
@@ -351,10 +366,10 @@ namespace mallocMC
                 if(chunksize <= hierarchyThreshold)
                 {
                     const u32 segmentsize = chunksize * 32u + 4u;
-                    const u32 full = umin32(32u, pagesize / segmentsize);
+                    const u32 full = umin32(32u, pdiv(pagesize, segmentsize));
                     return full * 32u + calcAdditional(full, segmentsize, chunksize);
                 }
-                return umin32(pagesize / chunksize, 32u);
+                return umin32(pdiv(pagesize, chunksize), 32u);
             }
 
             // ------------------------------------------- chunked path
@@ -809,6 +824,11 @@ namespace mallocMC
             h.accessblocks = g.accessblocks;
             h.pagesperblock = g.pagesperblock;
             h.flags = c.flags;
+            h.divR = FastDiv(c.regionsize);
+            h.divRegions = FastDiv(g.numregions);
+            h.divPpb = FastDiv(g.pagesperblock);
+            h.divP = FastDiv(c.pagesize);
+            h.divSlice8 = FastDiv(g.pagesperblock / 8u);
             return h;
         }
     } // namespace detail

@@ -18,8 +18,9 @@
  *   * frees are grouped by page: one atomicAnd per touched mask word, one
  *     atomicAnd on the segment hints and one atomicSub(count, k).
  *
- * The metadata protocol is the reference's (same PTE fields, on-page masks,
- * reset lock at count == pagesize, region fill levels, access-block cursor),
+ * The metadata protocol is the reference's (same PTE fields, segment masks --
+ * kept in a side array here --, reset lock at count == pagesize, region fill
+ * levels, access-block cursor),
  * so per-lane calls (scatter_heap.hpp) and wave calls can be mixed on one
  * heap.  Requests above a page and coalesced XMallocSIMD slices use the
  * per-lane path.
@@ -135,17 +136,15 @@ namespace mallocMC
                 }
                 return pos;
             }
-            //! The lowest `n` set bits of x.
+            //! The lowest `n` set bits of x (all of x if it has n or fewer).
             MALLOCMC_D u32 lowestBits(u32 x, u32 n)
             {
-                u32 out = 0u;
-                for(u32 i = 0; i < n && x != 0u; ++i)
-                {
-                    const u32 b = x & (0u - x);
-                    out |= b;
-                    x ^= b;
-                }
-                return out;
+                if(n == 0u)
+                    return 0u;
+                if(static_cast<u32>(hw::popc(x)) <= n)
+                    return x;
+                const u32 pos = selectBit(x, n - 1u); // position of the n-th set bit
+                return pos >= 31u ? x : (x & ((2u << pos) - 1u));
             }
         } // namespace wave
 
@@ -525,22 +524,32 @@ namespace mallocMC
                 const u32 me = hw::laneid();
                 const u32 R = h.regionsize;
                 const u32 ppb = h.pagesperblock;
-                const u32 numregions = h.numpages / R;
+                const u32 numregions = h.numregions;
                 const u32 hashBase = c * h.hashingK + h.hashingDistMP * hw::cuid()
-                    + (h.hashingDistWP + h.hashingDistWPRel * (64u * c / h.pagesize)) * hw::waveslot();
+                    + (h.hashingDistWP + h.hashingDistWPRel * h.divP.div(64u * c)) * hw::waveslot();
                 u32 startInBlock;
                 if(t.xcdSlices > 1u && ppb >= t.xcdSlices * R)
                 {
-                    const u32 slice = ppb / t.xcdSlices;
-                    startInBlock = (hw::xccid() % t.xcdSlices) * slice + hashBase % slice;
+                    if(t.xcdSlices == 8u)
+                        startInBlock = (hw::xccid() & 7u) * h.divSlice8.d + h.divSlice8.mod(hashBase);
+                    else
+                    {
+                        const u32 slice = ppb / t.xcdSlices;
+                        startInBlock = (hw::xccid() % t.xcdSlices) * slice + hashBase % slice;
+                    }
                 }
                 else
-                    startInBlock = hashBase % ppb;
+                    startInBlock = h.divPpb.mod(hashBase);
                 const u32 globalStart = startInBlock + ffb * ppb;
-                const u32 region0 = globalStart / R;
-                const u32 off0 = globalStart % R; // every region is entered at this page
+                const u32 region0 = h.divR.div(globalStart);
+                const u32 off0 = globalStart - region0 * R; // every region is entered at this page
+                // region0 + x for 0 <= x < 2 * numregions, wrapped
+                auto wrap = [numregions](u32 r) {
+                    r = r >= numregions ? r - numregions : r;
+                    return r >= numregions ? r - numregions : r;
+                };
                 // regions examined per round trip
-                const u32 W = R >= 64u ? 1u : 64u / R;
+                const u32 W = R >= 64u ? 1u : h.divR.div(64u);
                 const u32 span = R >= 64u ? 64u : W * R;
                 u64 need = group;
                 u32 checklevel = R * 3u / 4u;
@@ -554,13 +563,13 @@ namespace mallocMC
                     {
                         const u32 nreg = umin32(W, numregions - done);
                         // lane -> (region of this round, page within region)
-                        const u32 ri = R >= 64u ? 0u : me / R;
-                        const u32 region = (region0 + done + ri) % numregions;
+                        const u32 ri = R >= 64u ? 0u : h.divR.div(me);
+                        const u32 region = wrap(region0 + done + ri);
                         u32 fill = 0xffffffffu, pcs = 0u, pcount = 0u, pbits = 0u;
                         const bool inSpan = me < span && ri < nreg;
                         for(u32 base = 0; base < (R >= 64u ? R : 1u) && need != 0u; base += 64u)
                         {
-                            const u32 pj = R >= 64u ? base + me : me % R;
+                            const u32 pj = R >= 64u ? base + me : me - ri * R;
                             const bool valid = inSpan && pj < R;
                             if(valid)
                             {
@@ -591,7 +600,7 @@ namespace mallocMC
                                 if((open & sel) == 0u)
                                     continue; // region filled past the threshold
                                 const u32 fillK = wave::lane(fill, lo);
-                                const u32 regionK = R >= 64u ? region : (region0 + done + k) % numregions;
+                                const u32 regionK = R >= 64u ? region : wrap(region0 + done + k);
                                 u64 used = usedAll & sel;
                                 u64 cands = freeAll & sel;
                                 const u32 rotAbs = R >= 64u ? (off0 >= base ? off0 - base : 0u) : off0;
@@ -639,9 +648,10 @@ namespace mallocMC
                         u32 abid = 0u;
                         if(me < nreg)
                         {
-                            const u32 rg = (region0 + done + me + 1u) % numregions;
-                            if((rg * R) % ppb == 0u)
-                                abid = rg * R / ppb;
+                            const u32 rg = wrap(region0 + done + me + 1u);
+                            const u32 q = h.divPpb.div(rg * R);
+                            if(q * ppb == rg * R)
+                                abid = q;
                         }
                         abid = wave::max(abid);
                         if(abid != 0u && me == 0u && abid > atom::load(&h.scalars->firstfreeblock))
@@ -757,14 +767,13 @@ namespace mallocMC
                     // cap the group at the aggregation width
                     if(t.aggregation < 64u && hw::popc(group) > t.aggregation)
                     {
-                        u64 g = 0u, tmp = group;
-                        for(u32 i = 0; i < t.aggregation; ++i)
-                        {
-                            const u64 lo = tmp & (0u - tmp);
-                            g |= lo;
-                            tmp ^= lo;
-                        }
-                        group = g;
+                        // the lowest `aggregation` lanes of the group
+                        const u32 lo = static_cast<u32>(group), hi = static_cast<u32>(group >> 32);
+                        const u32 nlo = static_cast<u32>(hw::popc(lo));
+                        group = t.aggregation <= nlo
+                            ? static_cast<u64>(wave::lowestBits(lo, t.aggregation))
+                            : (static_cast<u64>(lo)
+                               | (static_cast<u64>(wave::lowestBits(hi, t.aggregation - nlo)) << 32));
                     }
                     const bool member = ((group >> me) & 1u) != 0u;
                     const u32 c = wave::max(member ? minAlloc : 0u);
@@ -794,7 +803,7 @@ namespace mallocMC
                 const u32 me = hw::laneid();
                 const u32 P = h.pagesize;
                 char* p = static_cast<char*>(pv);
-                u32 page = 0u, cs = 0u, inpage = 0u;
+                u32 page = 0u, cs = 0u, inpage = 0u, q = 0u;
                 bool simple = false;
                 if(p != nullptr)
                 {
@@ -803,11 +812,12 @@ namespace mallocMC
                     inpage = static_cast<u32>(p - h.pageData(page));
                     const bool chunked = cs != 0u
                         && (cs < P || (cs == P && (h.flags & kFixPagesizeLeak)));
-                    simple = chunked && (inpage % cs) == 0u;
+                    q = chunked ? h.pdiv(inpage, cs) : 0u;
+                    simple = chunked && q * cs == inpage;
                     if(!simple)
                         h.destroy<WaveIds>(p); // multi-page, XMallocSIMD slice, stale pointer
                 }
-                const u32 idx = simple ? inpage / cs : 0u;
+                const u32 idx = simple ? q : 0u;
                 const bool hier = simple && cs <= h.hierarchyThreshold;
                 const u32 seg = hier ? idx / 32u : 0u;
 
@@ -871,10 +881,10 @@ namespace mallocMC
                     const u32 thr = umax32(P / 2u / cs, 1u);
                     if(old >= thr && old - pageK < thr)
                     {
-                        u32* region = &h.regions[page / h.regionsize];
+                        u32* region = &h.regions[h.divR.div(page)];
                         if(atom::load(region) != 0u)
                             atom::exch(region, 0u);
-                        const u32 block = page / h.pagesperblock;
+                        const u32 block = h.divPpb.div(page);
                         if(block < atom::load(&h.scalars->firstfreeblock))
                             atom::min(&h.scalars->firstfreeblock, block);
                     }

@@ -31,6 +31,72 @@ namespace mallocMC
         return (a + b - 1) / b;
     }
 
+    //! Unsigned 32-bit division by a divisor fixed at heap set-up, as a
+    //! multiply-high and shifts (Granlund & Montgomery, "Division by
+    //! invariant integers using multiplication", PLDI 1994, the round-up
+    //! variant with the added bit): exact for every 32-bit dividend.
+    struct FastDiv
+    {
+        std::uint32_t d = 1u;
+        std::uint32_t m = 0u; // 0: d is a power of two, q = n >> s
+        std::uint32_t s = 0u;
+
+        MALLOCMC_HD FastDiv() = default;
+
+        MALLOCMC_HD explicit FastDiv(std::uint32_t divisor)
+        {
+            d = divisor == 0u ? 1u : divisor;
+            if((d & (d - 1u)) == 0u)
+            {
+                m = 0u;
+                s = 0u;
+                while((1u << s) < d)
+                    ++s;
+            }
+            else
+            {
+                s = 0u;
+                while((std::uint64_t{1} << s) < d)
+                    ++s; // 2^(s-1) < d < 2^s
+                const std::uint64_t num = (std::uint64_t{1} << 32) * ((std::uint64_t{1} << s) - d);
+                m = static_cast<std::uint32_t>(num / d + 1u);
+            }
+        }
+
+        MALLOCMC_HD std::uint32_t div(std::uint32_t n) const
+        {
+            if(m == 0u)
+                return n >> s;
+            const std::uint32_t t = static_cast<std::uint32_t>((static_cast<std::uint64_t>(n) * m) >> 32);
+            return (t + ((n - t) >> 1)) >> (s - 1u);
+        }
+
+        MALLOCMC_HD std::uint32_t mod(std::uint32_t n) const
+        {
+            return n - div(n) * d;
+        }
+    };
+
+    //! n / d for n < 2^24 and d >= 1 with a float reciprocal and one
+    //! correction each way.  Exact whenever the quotient is below 2^22 (the
+    //! reciprocal and the product together err by less than 2^-22 relative,
+    //! so the first quotient is within one of the exact one); larger
+    //! quotients may be off by a few units, which callers that clamp the
+    //! result to 32 or less never see.
+    MALLOCMC_D std::uint32_t udiv24(std::uint32_t n, std::uint32_t d)
+    {
+        std::uint32_t q = static_cast<std::uint32_t>(static_cast<float>(n) * __builtin_amdgcn_rcpf(static_cast<float>(d)));
+        int r = static_cast<int>(n) - static_cast<int>(q * d);
+        if(r < 0)
+        {
+            --q;
+            r += static_cast<int>(d);
+        }
+        if(r >= static_cast<int>(d))
+            ++q;
+        return q;
+    }
+
     namespace hw
     {
         MALLOCMC_D std::uint32_t laneid()

@@ -54,6 +54,14 @@ int mmc_bench_copy(const void* d_src, void* d_dst, uint64_t bytes, void* stream)
  * (uint32: max, min, or, prefix; zeroed by the call). */
 int mmc_selftest_wave(uint32_t waves, uint64_t seed, uint32_t* d_result, void* stream);
 
+/* Host self-test of the division by invariant divisors the heap view uses
+ * (regionsize, numregions, pages per access block, pagesize), no reference
+ * counterpart: `count` seeded (dividend, divisor) pairs plus edge cases
+ * (powers of two and their neighbours, 0 and 2^32-1 dividends, multiples of
+ * the divisor and their neighbours) against the `/` operator.  Returns the
+ * number of mismatches in *mismatches.  Runs on the host only. */
+int mmc_selftest_fastdiv(uint64_t seed, uint64_t count, uint64_t* mismatches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -263,6 +263,52 @@ int mmc_selftest_wave(uint32_t waves, uint64_t seed, uint32_t* d_result, void* s
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "selftest launch");
 }
 
+int mmc_selftest_fastdiv(uint64_t seed, uint64_t count, uint64_t* mismatches)
+{
+    if(!mismatches)
+        return mmc::fail(MMC_EINVAL, "null output");
+    uint64_t bad = 0;
+    uint64_t z = seed;
+    auto next = [&z]() {
+        z += 0x9E3779B97F4A7C15ull;
+        uint64_t x = z;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        return x ^ (x >> 31);
+    };
+    auto check = [&bad](uint32_t n, uint32_t d) {
+        const mallocMC::FastDiv f(d);
+        if(f.div(n) != n / d || f.mod(n) != n % d)
+            ++bad;
+    };
+    for(uint32_t k = 0; k < 32u; ++k)
+        for(int64_t dd = -2; dd <= 2; ++dd)
+        {
+            const int64_t d64 = (int64_t{1} << k) + dd;
+            if(d64 < 1 || d64 > 0xffffffffll)
+                continue;
+            const uint32_t d = static_cast<uint32_t>(d64);
+            const uint32_t edges[] = {0u, 1u, d - 1u, d, d + 1u, 0x7fffffffu, 0x80000000u, 0xfffffffeu, 0xffffffffu,
+                                      static_cast<uint32_t>((0xffffffffull / d) * d),
+                                      static_cast<uint32_t>((0xffffffffull / d) * d - 1u)};
+            for(uint32_t n : edges)
+                check(n, d);
+        }
+    for(uint64_t i = 0; i < count; ++i)
+    {
+        const uint64_t r = next();
+        // divisors spread over all magnitudes
+        const uint32_t shift = static_cast<uint32_t>(r >> 59); // 0..31
+        uint32_t d = static_cast<uint32_t>(r) >> shift;
+        d = d == 0u ? 1u : d;
+        const uint32_t n = static_cast<uint32_t>(next());
+        check(n, d);
+        check(static_cast<uint32_t>((static_cast<uint64_t>(n / d) * d)), d);
+    }
+    *mismatches = bad;
+    return MMC_OK;
+}
+
 int mmc_bench_copy(const void* d_src, void* d_dst, uint64_t bytes, void* stream)
 {
     if(!d_src || !d_dst || (bytes % 16u) != 0u)

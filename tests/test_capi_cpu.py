@@ -151,3 +151,13 @@ def test_serve_trace_shape():
     assert (np.diff(t.arrival.astype(np.int64)) >= 0).all()
     assert t.length.min() >= 1 and t.length.max() <= 8192
     assert abs(len(t.arrival) / 30.0 - 2000) < 100  # Poisson 2000/s over 30 s
+
+
+def test_fastdiv_matches_division(mmc):
+    """The heap view's division by invariant divisors (multiply-high and
+    shifts) equals `/` and `%` on edge cases and 2,000,000 seeded pairs
+    (host run of the same header code the kernels use)."""
+    import ctypes
+    bad = ctypes.c_uint64(123)
+    assert mmc.lib().mmc_selftest_fastdiv(7, 1_000_000, ctypes.byref(bad)) == 0
+    assert bad.value == 0
