@@ -377,8 +377,12 @@ namespace mallocMC
                     {
                         const u32 old = atom::or_(&words[me], choose);
                         gotBits = choose & ~old;
-                        if(hier && ((old | choose) & sel) == sel)
-                            atom::or_(h.pteBitmask(page), 1u << me);
+                        // The reference also marks a segment it filled as full in
+                        // the PTE bitmask (addChunkHierarchy, Scatter.hpp:408-419).
+                        // The wave reads every mask word anyway, so it sets no
+                        // such hint: a hint set after a concurrent free of that
+                        // segment would outlive the page's chunks (a stale bit the
+                        // next flat chunk size would read as a chunk in use).
                     }
                     // hand out the chunks word by word, in rank order
                     u64 words64 = hw::ballot(gotBits != 0u);

@@ -22,6 +22,7 @@ SOURCES = [
     "mallocmc_harness.hip",
     "mallocmc_batched.hip",
     "mallocmc_serve.hip",
+    "mallocmc_snapshot.hip",
 ]
 
 HIPCC_FLAGS = [
