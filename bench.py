@@ -239,6 +239,16 @@ def main():
             traffic = json.load(f).get("waveMallocKernel<NoopDistribution>", {}).get("hbm_bytes_per_launch")
     achieved_gbs = N_LANES * MALLOC_BYTES_PER_REQUEST / (mal_avg_ms / 1e3) / 1e9
     atom_rate = N_LANES * MALLOC_RMW_PER_REQUEST / (mal_avg_ms / 1e3)
+    # the reference protocol's read-modify-writes per malloc+free pair, uncontended,
+    # with resetfreedpages: malloc = chunk-size CAS + count add + mask or + chunk-size
+    # re-CAS (Scatter.hpp:469-517, 589-591); free = mask and (+ segment-hint and on
+    # a hierarchical page) + failed count CAS + count sub (Scatter.hpp:646-716):
+    # 8 for chunks up to the hierarchy threshold, 7 above it
+    ht = (best.pagesize - 8) // 33
+    sz = np.concatenate([b.cpu().numpy() for b in batches])
+    frac_h = float(np.mean(((sz + 15) // 16 * 16) <= ht))
+    rmw_pair = 8.0 * frac_h + 7.0 * (1.0 - frac_h)
+    pair_rate = value / 2.0 * rmw_pair
 
     # copy kernel of known size: calibrates the HBM byte counters of a profile run
     copy_bytes = 256 * 1024 * 1024
@@ -314,6 +324,12 @@ def main():
         "atomic_roofline": {
             "achieved": atom_rate, "peak": ceiling, "unit": "reference-protocol RMW/s",
             "frac": atom_rate / ceiling, "rmw_per_request": MALLOC_RMW_PER_REQUEST,
+            "per_pair": {
+                "achieved": pair_rate, "frac": pair_rate / ceiling, "rmw_per_pair": rmw_pair,
+                "hierarchical_fraction": frac_h,
+                "note": "whole step (malloc+free pairs/s) x the reference protocol's RMW per pair; "
+                        "the wave path issues fewer RMW than that, so frac may exceed 1",
+            },
         },
         "cpu_baseline": cpu,
         "serve": serve,

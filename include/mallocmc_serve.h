@@ -15,8 +15,10 @@
  *   3. completion: a request that reached its length reads its whole buffer
  *      back into a 64-bit checksum, frees its blocks (wave-cooperative free)
  *      and returns its slot and reservation.
- * The three launches of a step can be captured in one graph; the step counter
- * lives in device memory.  Payload word w of token t of request r is
+ * A step is two launches: decode, then completion, whose last workgroup
+ * also runs the admission for the next step (the first admission runs at
+ * load).  Both can be captured in one graph; the step counter lives in
+ * device memory.  Payload word w of token t of request r is
  * mmc_serve_word(r, t, w) below, so every checksum has a closed form.
  */
 #ifndef MALLOCMC_SERVE_H

@@ -27,12 +27,12 @@ namespace
 
     struct ServeScalars
     {
-        u64 step;
+        u64 step;     // admission rounds done (the next admission serves arrivals <= step)
         u64 reserved;
         u32 cursor;
         u32 freeTop;
         u32 nDone;
-        u32 pad;
+        u32 ticket;   // completion blocks finished this step (self-resetting)
         u64 counters[cCount];
     };
 
@@ -45,6 +45,7 @@ namespace
         u32* slotReq;
         u32* slotLen;
         u32* slotCur;
+        u32* chunkDone; // checksum chunks finished per slot (self-resetting)
         u64* slotBlocks;
         u32* doneList;
         const u32* arrival;
@@ -70,47 +71,50 @@ namespace
         return x;
     }
 
-    //! Step part 1, one lane: admit arrived requests in order while a slot and
-    //! budget are available.
-    __global__ void admitKernel(ServeView v)
+    //! One lane: admit the requests that have arrived by step sc->step, in
+    //! order, while a slot and budget are available; then advance the step.
+    //! Runs at the end of the previous step's completion kernel (or at load
+    //! for step 0), after every slot and reservation of that step came back.
+    //! Slot state is written with device-scope atomics: the completion
+    //! kernel's other blocks wrote it in the same launch from other XCDs.
+    __device__ void admit(const ServeView& v)
     {
-        if(blockIdx.x != 0 || threadIdx.x != 0)
-            return;
         ServeScalars* sc = v.sc;
-        sc->nDone = 0u;
-        const u64 step = sc->step;
-        u32 cursor = sc->cursor;
-        u64 reserved = sc->reserved;
-        u32 top = sc->freeTop;
+        const u64 step = atom::load(&sc->step);
+        u32 cursor = atom::load(&sc->cursor);
+        u64 reserved = atom::load(&sc->reserved);
+        u32 top = atom::load(&sc->freeTop);
+        u64 admitted = 0u, wait = 0u;
         while(cursor < v.nRequests && v.arrival[cursor] <= step)
         {
             const u32 len = v.length[cursor];
             const u64 need = static_cast<u64>(divup(len, v.tokensPerBlock)) * v.blockBytes;
             if(reserved + need > v.budget || top == 0u)
                 break;
-            const u32 slot = v.freeStack[--top];
-            v.slotReq[slot] = cursor;
-            v.slotLen[slot] = len;
-            v.slotCur[slot] = 0u;
-            v.checksums[cursor] = 0u;
+            const u32 slot = atom::load(&v.freeStack[--top]);
+            atom::store(&v.slotLen[slot], len);
+            atom::store(&v.slotCur[slot], 0u);
+            atom::store(&v.slotReq[slot], cursor);
+            atom::exch(&v.checksums[cursor], u64{0});
             reserved += need;
-            sc->counters[cAdmitted] += 1u;
-            sc->counters[cQueueWait] += step - v.arrival[cursor];
+            ++admitted;
+            wait += step - v.arrival[cursor];
             if(v.admitStep)
                 v.admitStep[cursor] = static_cast<u32>(step);
             ++cursor;
         }
-        sc->cursor = cursor;
-        sc->reserved = reserved;
-        sc->freeTop = top;
+        atom::store(&sc->cursor, cursor);
+        atom::exch(&sc->reserved, reserved);
+        atom::store(&sc->freeTop, top);
+        atom::add(&sc->counters[cAdmitted], admitted);
+        atom::add(&sc->counters[cQueueWait], wait);
         const u64 active = v.maxSlots - top;
-        sc->counters[cActive] = active;
-        if(active > sc->counters[cMaxActive])
-            sc->counters[cMaxActive] = active;
-        sc->step = step + 1u;
+        atom::exch(&sc->counters[cActive], active);
+        atom::max(&sc->counters[cMaxActive], active);
+        atom::exch(&sc->step, step + 1u);
     }
 
-    //! Step part 2, one lane per request slot: append one token.
+    //! Step part 1, one lane per request slot: append one token.
     __global__ __launch_bounds__(256) void decodeKernel(ServeView v)
     {
         const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -169,14 +173,51 @@ namespace
     }
 
     constexpr u32 kTokensPerChunk = 64u;
+    constexpr u32 kCompleteBlocks = 1024u;
 
-    //! Step part 3a: the paged buffers of the finished requests, cut into
-    //! 64-token chunks spread over the whole grid, are read back into their
-    //! checksums (lane = word of a token, the 4 waves of a workgroup take every
-    //! fourth token, 4 tokens in flight per lane; partial sums add atomically).
-    __global__ __launch_bounds__(256) void checksumKernel(ServeView v)
+    //! Returns a finished request's blocks (wave-cooperative free), slot and
+    //! reservation; the whole wave calls it.  The results that the admission
+    //! at the end of the launch reads are returning atomics, waited for here.
+    __device__ void release(const ServeView& v, u32 slot)
+    {
+        const u32 lane = hw::laneid();
+        const WaveAlloc wa{v.heap, WaveTuning{}};
+        const u32 nb = divup(atom::load(&v.slotLen[slot]), v.tokensPerBlock);
+        const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
+        u32 freed = 0u;
+        for(u32 b = 0; b < nb; b += 64u)
+        {
+            const u32 j = b + lane;
+            const u64 off = j < nb ? blocks[j] : kNullOffset;
+            wa.free(off == kNullOffset ? nullptr : v.heap.pages + off);
+            freed += off != kNullOffset ? 1u : 0u;
+        }
+        for(int d = 32; d > 0; d >>= 1)
+            freed += __shfl_xor(freed, d, 64);
+        if(lane == 0u)
+        {
+            atom::add(&v.sc->counters[cCompleted], u64{1});
+            atom::add(&v.sc->counters[cFrees], static_cast<u64>(freed));
+            const u64 r = atom::sub(&v.sc->reserved, static_cast<u64>(nb) * v.blockBytes);
+            const u32 q = atom::exch(&v.slotReq[slot], kEmpty);
+            atom::store(&v.chunkDone[slot], 0u);
+            const u32 pos = atom::add(&v.sc->freeTop, 1u);
+            const u32 e = atom::exch(&v.freeStack[pos], slot);
+            hw::consume(static_cast<u32>(r) ^ q ^ e);
+        }
+    }
+
+    //! Step part 2, completion: the paged buffers of the finished requests,
+    //! cut into 64-token chunks spread over the whole grid, are read back into
+    //! their checksums (lane = word of a token, the 4 waves of a workgroup take
+    //! every fourth token, 4 tokens in flight per lane; partial sums add
+    //! atomically).  The workgroup that finishes a request's last chunk
+    //! releases the request; the last workgroup to finish resets the done list
+    //! and admits the next step's arrivals, so one step is two launches.
+    __global__ __launch_bounds__(256) void completeKernel(ServeView v)
     {
         __shared__ u64 partial[4];
+        __shared__ u32 lastChunk;
         const u32 lane = hw::laneid();
         const u32 q = threadIdx.x / 64u;
         const u32 nDone = atom::load(&v.sc->nDone);
@@ -238,44 +279,23 @@ namespace
                 partial[q] = cs;
             __syncthreads();
             if(threadIdx.x == 0u)
+            {
                 atom::add(&v.checksums[req], partial[0] + partial[1] + partial[2] + partial[3]);
+                // every read of this chunk is done (the loads returned before
+                // the sums): the request's last chunk may release its blocks
+                lastChunk = atom::add(&v.chunkDone[slot], 1u) + 1u == nChunks ? 1u : 0u;
+            }
+            __syncthreads();
+            if(lastChunk && q == 0u)
+                release(v, slot);
             __syncthreads();
         }
-    }
-
-    //! Step part 3b, one wave per finished request: free its blocks, return
-    //! slot and reservation.
-    __global__ __launch_bounds__(256) void releaseKernel(ServeView v)
-    {
-        const u32 wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
-        const u32 nWaves = (gridDim.x * blockDim.x) / 64u;
-        const u32 lane = hw::laneid();
-        const u32 nDone = atom::load(&v.sc->nDone);
-        const WaveAlloc wa{v.heap, WaveTuning{}};
-        for(u32 i = wave; i < nDone; i += nWaves)
+        if(threadIdx.x == 0u && atom::add(&v.sc->ticket, 1u) == gridDim.x - 1u)
         {
-            const u32 slot = v.doneList[i];
-            const u32 nb = divup(v.slotLen[slot], v.tokensPerBlock);
-            const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
-            u32 freed = 0u;
-            for(u32 b = 0; b < nb; b += 64u)
-            {
-                const u32 j = b + lane;
-                const u64 off = j < nb ? blocks[j] : kNullOffset;
-                wa.free(off == kNullOffset ? nullptr : v.heap.pages + off);
-                freed += off != kNullOffset ? 1u : 0u;
-            }
-            for(int d = 32; d > 0; d >>= 1)
-                freed += __shfl_xor(freed, d, 64);
-            if(lane == 0u)
-            {
-                atom::add(&v.sc->counters[cCompleted], u64{1});
-                atom::add(&v.sc->counters[cFrees], static_cast<u64>(freed));
-                atom::sub(&v.sc->reserved, static_cast<u64>(nb) * v.blockBytes);
-                v.slotReq[slot] = kEmpty;
-                const u32 pos = atom::add(&v.sc->freeTop, 1u);
-                v.freeStack[pos] = slot;
-            }
+            // last workgroup: every release of this step is done
+            atom::store(&v.sc->ticket, 0u);
+            atom::store(&v.sc->nDone, 0u);
+            admit(v);
         }
     }
 
@@ -287,6 +307,7 @@ namespace
         {
             v.freeStack[i] = v.maxSlots - 1u - i;
             v.slotReq[i] = kEmpty;
+            v.chunkDone[i] = 0u;
         }
         if(tid == 0u)
         {
@@ -295,9 +316,17 @@ namespace
             v.sc->cursor = 0u;
             v.sc->freeTop = v.maxSlots;
             v.sc->nDone = 0u;
+            v.sc->ticket = 0u;
             for(u32 c = 0; c < cCount; ++c)
                 v.sc->counters[c] = 0u;
         }
+    }
+
+    //! Admission for step 0 after the reset.
+    __global__ void firstAdmitKernel(ServeView v)
+    {
+        if(blockIdx.x == 0u && threadIdx.x == 0u)
+            admit(v);
     }
 } // namespace
 
@@ -325,7 +354,7 @@ int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** ou
     s->cfg = *cfg;
     const u32 maxBlocks = divup(cfg->max_tokens, cfg->tokens_per_block);
     const u64 n = cfg->max_slots;
-    const u64 bytes = sizeof(ServeScalars) + 4u * n * 5u + 8u * n * maxBlocks + 256u;
+    const u64 bytes = sizeof(ServeScalars) + 4u * n * 6u + 8u * n * maxBlocks + 256u;
     hipError_t err = hipMalloc(&s->arena, bytes);
     if(err != hipSuccess)
     {
@@ -348,6 +377,8 @@ int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** ou
     v.slotCur = reinterpret_cast<u32*>(a);
     a += 4u * n;
     v.doneList = reinterpret_cast<u32*>(a);
+    a += 4u * n;
+    v.chunkDone = reinterpret_cast<u32*>(a);
     v.maxSlots = cfg->max_slots;
     v.blockBytes = cfg->block_bytes;
     v.tokensPerBlock = cfg->tokens_per_block;
@@ -381,6 +412,7 @@ int mmc_serve_load(mmc_serve* s, const uint32_t* d_arrival, const uint32_t* d_le
     s->view.admitStep = d_admit_step;
     s->view.nRequests = n;
     hipLaunchKernelGGL(resetKernel, dim3(64), dim3(256), 0, static_cast<hipStream_t>(stream), s->view);
+    hipLaunchKernelGGL(firstAdmitKernel, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), s->view);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve reset launch");
 }
@@ -390,10 +422,8 @@ int mmc_serve_step(mmc_serve* s, void* stream)
     if(!s)
         return mmc::fail(MMC_EINVAL, "null argument");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(admitKernel, dim3(1), dim3(64), 0, st, s->view);
     hipLaunchKernelGGL(decodeKernel, dim3(mmc::gridFor(s->cfg.max_slots, 256)), dim3(256), 0, st, s->view);
-    hipLaunchKernelGGL(checksumKernel, dim3(1024), dim3(256), 0, st, s->view);
-    hipLaunchKernelGGL(releaseKernel, dim3(128), dim3(256), 0, st, s->view);
+    hipLaunchKernelGGL(completeKernel, dim3(kCompleteBlocks), dim3(256), 0, st, s->view);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve step launch");
 }
@@ -409,7 +439,7 @@ int mmc_serve_get_stats(mmc_serve* s, mmc_serve_stats* out, void* stream)
         err = hipStreamSynchronize(st);
     if(err != hipSuccess)
         return mmc::hipFail(err, "serve stats copy");
-    out->step = h.step;
+    out->step = h.step > 0u ? h.step - 1u : 0u; // admission rounds include the one at load
     out->admitted = h.counters[cAdmitted];
     out->completed = h.counters[cCompleted];
     out->block_mallocs = h.counters[cMallocs];
