@@ -80,8 +80,24 @@ int mmc_serve_destroy(mmc_serve* s);
 int mmc_serve_load(mmc_serve* s, const uint32_t* d_arrival, const uint32_t* d_length, const uint32_t* d_ids,
                    uint32_t n, uint64_t* d_checksums, uint32_t* d_admit_step, void* stream);
 
-/* Enqueues one step (admission, decode, completion) on `stream`. */
+/* Enqueues one step (decode, then completion with the next admission) on
+ * `stream`.  With a timestamp buffer attached, the step first records the
+ * device clock into the next entry of that buffer. */
 int mmc_serve_step(mmc_serve* s, void* stream);
+
+/* Attaches a device buffer of `capacity` uint64 device-clock stamps (NULL
+ * detaches): every later step, and each mmc_serve_stamp call, appends one
+ * stamp at its start, so step i took stamps[i + 1] - stamps[i] clock ticks
+ * on the device, with no host launch in between when the steps are replayed
+ * from one graph.  Stamps beyond `capacity` are dropped; the count restarts
+ * at mmc_serve_load.  *clock_hz receives the clock's frequency. */
+int mmc_serve_timestamps(mmc_serve* s, uint64_t* d_stamps, uint32_t capacity, uint64_t* clock_hz);
+
+/* Appends one stamp (e.g. after the last step) on `stream`. */
+int mmc_serve_stamp(mmc_serve* s, void* stream);
+
+/* Number of stamps recorded since mmc_serve_load (synchronizes `stream`). */
+int mmc_serve_stamp_count(mmc_serve* s, uint32_t* count, void* stream);
 
 /* Copies the counters to the host (synchronizes `stream`). */
 int mmc_serve_get_stats(mmc_serve* s, mmc_serve_stats* out, void* stream);

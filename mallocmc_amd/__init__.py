@@ -42,6 +42,7 @@ BATCHED_SYMBOLS = [
     "mmc_multi_destroy", "mmc_multi_step", "mmc_wave_fill_until_null",
     # include/mallocmc_serve.h, bound in mallocmc_amd/serve.py
     "mmc_serve_create", "mmc_serve_destroy", "mmc_serve_load", "mmc_serve_step", "mmc_serve_get_stats",
+    "mmc_serve_timestamps", "mmc_serve_stamp", "mmc_serve_stamp_count",
     # include/mallocmc_snapshot.h, bound in mallocmc_amd/snapshot.py
     "mmc_snapshot_bytes", "mmc_snapshot", "mmc_snapshot_verify",
 ]

@@ -34,6 +34,8 @@ namespace
         u32 nDone;
         u32 ticket;   // completion blocks finished this step (self-resetting)
         u64 counters[cCount];
+        u32 nStamps;  // device-clock stamps recorded since load
+        u32 pad;
     };
 
     //! Everything a serving kernel needs, passed by value.
@@ -71,47 +73,87 @@ namespace
         return x;
     }
 
-    //! One lane: admit the requests that have arrived by step sc->step, in
-    //! order, while a slot and budget are available; then advance the step.
-    //! Runs at the end of the previous step's completion kernel (or at load
-    //! for step 0), after every slot and reservation of that step came back.
-    //! Slot state is written with device-scope atomics: the completion
-    //! kernel's other blocks wrote it in the same launch from other XCDs.
-    __device__ void admit(const ServeView& v)
+    //! One wave: admit the requests that have arrived by step sc->step, in
+    //! arrival order, while a slot and budget are available; then advance
+    //! the step.  Runs at the end of the previous step's completion kernel (or
+    //! at load for step 0), after every slot and reservation of that step came
+    //! back.  Up to 64 candidates per round: their projected bytes are summed
+    //! with a prefix scan, the leading run that fits the budget and the free
+    //! slots is admitted at once, so a round costs a few round trips however
+    //! many requests it admits.  Slot state is written with device-scope
+    //! atomics: the completion kernel's other blocks wrote it in the same
+    //! launch from other XCDs.
+    __device__ void admitWave(const ServeView& v)
     {
         ServeScalars* sc = v.sc;
-        const u64 step = atom::load(&sc->step);
-        u32 cursor = atom::load(&sc->cursor);
-        u64 reserved = atom::load(&sc->reserved);
-        u32 top = atom::load(&sc->freeTop);
-        u64 admitted = 0u, wait = 0u;
-        while(cursor < v.nRequests && v.arrival[cursor] <= step)
+        const u32 lane = hw::laneid();
+        u64 step = 0u, reserved = 0u;
+        u32 cursor = 0u, top = 0u;
+        if(lane == 0u)
         {
-            const u32 len = v.length[cursor];
-            const u64 need = static_cast<u64>(divup(len, v.tokensPerBlock)) * v.blockBytes;
-            if(reserved + need > v.budget || top == 0u)
-                break;
-            const u32 slot = atom::load(&v.freeStack[--top]);
-            atom::store(&v.slotLen[slot], len);
-            atom::store(&v.slotCur[slot], 0u);
-            atom::store(&v.slotReq[slot], cursor);
-            atom::exch(&v.checksums[cursor], u64{0});
-            reserved += need;
-            ++admitted;
-            wait += step - v.arrival[cursor];
-            if(v.admitStep)
-                v.admitStep[cursor] = static_cast<u32>(step);
-            ++cursor;
+            step = atom::load(&sc->step);
+            cursor = atom::load(&sc->cursor);
+            reserved = atom::load(&sc->reserved);
+            top = atom::load(&sc->freeTop);
         }
-        atom::store(&sc->cursor, cursor);
-        atom::exch(&sc->reserved, reserved);
-        atom::store(&sc->freeTop, top);
-        atom::add(&sc->counters[cAdmitted], admitted);
-        atom::add(&sc->counters[cQueueWait], wait);
-        const u64 active = v.maxSlots - top;
-        atom::exch(&sc->counters[cActive], active);
-        atom::max(&sc->counters[cMaxActive], active);
-        atom::exch(&sc->step, step + 1u);
+        step = hw::readlane64(step, 0u);
+        reserved = hw::readlane64(reserved, 0u);
+        cursor = wave::lane(cursor, 0u);
+        top = wave::lane(top, 0u);
+        u64 admitted = 0u, wait = 0u;
+        for(;;)
+        {
+            const u32 idx = cursor + lane;
+            const bool arrived = idx < v.nRequests && v.arrival[idx] <= step;
+            const u32 len = arrived ? v.length[idx] : 0u;
+            const u32 need = arrived ? divup(len, v.tokensPerBlock) * v.blockBytes : 0u;
+            // inclusive prefix of the projected bytes over lanes 0..lane
+            u32 incl = need;
+            for(int d = 1; d < 64; d <<= 1)
+            {
+                const u32 o = __shfl_up(incl, static_cast<unsigned>(d), 64);
+                if(lane >= static_cast<u32>(d))
+                    incl += o;
+            }
+            const bool fits = arrived && reserved + incl <= v.budget && lane < top;
+            // the leading run of lanes that fit is admitted, in order
+            const u64 ok = hw::ballot(fits);
+            const u32 count = ~ok == 0u ? 64u : static_cast<u32>(__builtin_ctzll(~ok));
+            if(lane < count)
+            {
+                const u32 slot = atom::load(&v.freeStack[top - 1u - lane]);
+                atom::store(&v.slotLen[slot], len);
+                atom::store(&v.slotCur[slot], 0u);
+                atom::store(&v.slotReq[slot], idx);
+                atom::exch(&v.checksums[idx], u64{0});
+                if(v.admitStep)
+                    v.admitStep[idx] = static_cast<u32>(step);
+            }
+            const u32 waitLane = lane < count ? static_cast<u32>(step - v.arrival[idx]) : 0u;
+            u32 w = waitLane;
+            for(int d = 32; d > 0; d >>= 1)
+                w += __shfl_xor(w, d, 64);
+            const u32 bytes = count ? wave::lane(incl, count - 1u) : 0u;
+            reserved += bytes;
+            top -= count;
+            cursor += count;
+            admitted += count;
+            wait += w;
+            if(count < 64u)
+                break;
+        }
+        if(lane == 0u)
+        {
+            atom::store(&sc->cursor, cursor);
+            atom::exch(&sc->reserved, reserved);
+            atom::store(&sc->freeTop, top);
+            atom::add(&sc->counters[cAdmitted], admitted);
+            atom::add(&sc->counters[cQueueWait], wait);
+            const u64 active = v.maxSlots - top;
+            atom::exch(&sc->counters[cActive], active);
+            atom::max(&sc->counters[cMaxActive], active);
+            atom::exch(&sc->step, step + 1u);
+        }
     }
 
     //! Step part 1, one lane per request slot: append one token.
@@ -290,12 +332,19 @@ namespace
                 release(v, slot);
             __syncthreads();
         }
-        if(threadIdx.x == 0u && atom::add(&v.sc->ticket, 1u) == gridDim.x - 1u)
+        __shared__ u32 last;
+        if(threadIdx.x == 0u)
+            last = atom::add(&v.sc->ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
+        __syncthreads();
+        if(last && q == 0u)
         {
             // last workgroup: every release of this step is done
-            atom::store(&v.sc->ticket, 0u);
-            atom::store(&v.sc->nDone, 0u);
-            admit(v);
+            if(lane == 0u)
+            {
+                atom::store(&v.sc->ticket, 0u);
+                atom::store(&v.sc->nDone, 0u);
+            }
+            admitWave(v);
         }
     }
 
@@ -317,16 +366,27 @@ namespace
             v.sc->freeTop = v.maxSlots;
             v.sc->nDone = 0u;
             v.sc->ticket = 0u;
+            v.sc->nStamps = 0u;
             for(u32 c = 0; c < cCount; ++c)
                 v.sc->counters[c] = 0u;
         }
     }
 
-    //! Admission for step 0 after the reset.
+    //! Appends the device clock to the stamp buffer (one lane).
+    __global__ void stampKernel(ServeScalars* sc, u64* stamps, u32 capacity)
+    {
+        if(blockIdx.x != 0u || threadIdx.x != 0u)
+            return;
+        const u64 t = wall_clock64();
+        const u32 i = atom::add(&sc->nStamps, 1u);
+        if(i < capacity)
+            stamps[i] = t;
+    }
+
+    //! Admission for step 0 after the reset (one wave).
     __global__ void firstAdmitKernel(ServeView v)
     {
-        if(blockIdx.x == 0u && threadIdx.x == 0u)
-            admit(v);
+        admitWave(v);
     }
 } // namespace
 
@@ -336,6 +396,8 @@ struct mmc_serve
     mmc_serve_config cfg{};
     ServeView view{};
     void* arena = nullptr;
+    u64* stamps = nullptr;
+    u32 stampCapacity = 0;
 };
 
 extern "C" {
@@ -422,10 +484,54 @@ int mmc_serve_step(mmc_serve* s, void* stream)
     if(!s)
         return mmc::fail(MMC_EINVAL, "null argument");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if(s->stamps)
+        hipLaunchKernelGGL(stampKernel, dim3(1), dim3(64), 0, st, s->view.sc, s->stamps, s->stampCapacity);
     hipLaunchKernelGGL(decodeKernel, dim3(mmc::gridFor(s->cfg.max_slots, 256)), dim3(256), 0, st, s->view);
     hipLaunchKernelGGL(completeKernel, dim3(kCompleteBlocks), dim3(256), 0, st, s->view);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve step launch");
+}
+
+int mmc_serve_timestamps(mmc_serve* s, uint64_t* d_stamps, uint32_t capacity, uint64_t* clock_hz)
+{
+    if(!s)
+        return mmc::fail(MMC_EINVAL, "null argument");
+    s->stamps = capacity ? d_stamps : nullptr;
+    s->stampCapacity = d_stamps ? capacity : 0u;
+    if(clock_hz)
+    {
+        int khz = 0;
+        const hipError_t err = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->heap->device);
+        if(err != hipSuccess)
+            return mmc::hipFail(err, "hipDeviceGetAttribute(wall clock rate)");
+        *clock_hz = static_cast<uint64_t>(khz) * 1000u;
+    }
+    return MMC_OK;
+}
+
+int mmc_serve_stamp(mmc_serve* s, void* stream)
+{
+    if(!s || !s->stamps)
+        return mmc::fail(MMC_EINVAL, "no timestamp buffer attached");
+    hipLaunchKernelGGL(stampKernel, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), s->view.sc, s->stamps,
+                       s->stampCapacity);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "stamp launch");
+}
+
+int mmc_serve_stamp_count(mmc_serve* s, uint32_t* count, void* stream)
+{
+    if(!s || !count)
+        return mmc::fail(MMC_EINVAL, "null argument");
+    ServeScalars h{};
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemcpyAsync(&h, s->view.sc, sizeof(h), hipMemcpyDeviceToHost, st);
+    if(err == hipSuccess)
+        err = hipStreamSynchronize(st);
+    if(err != hipSuccess)
+        return mmc::hipFail(err, "stamp count copy");
+    *count = h.nStamps;
+    return MMC_OK;
 }
 
 int mmc_serve_get_stats(mmc_serve* s, mmc_serve_stats* out, void* stream)

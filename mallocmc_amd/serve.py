@@ -38,7 +38,8 @@ class MmcServeStats(ctypes.Structure):
         "reserved_bytes", "max_active")]
 
 
-SERVE_SYMBOLS = ["mmc_serve_create", "mmc_serve_destroy", "mmc_serve_load", "mmc_serve_step", "mmc_serve_get_stats"]
+SERVE_SYMBOLS = ["mmc_serve_create", "mmc_serve_destroy", "mmc_serve_load", "mmc_serve_step", "mmc_serve_get_stats",
+                 "mmc_serve_timestamps", "mmc_serve_stamp", "mmc_serve_stamp_count"]
 
 
 def _bind():
@@ -52,6 +53,9 @@ def _bind():
     L.mmc_serve_load.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp]
     L.mmc_serve_step.argtypes = [vp, vp]
     L.mmc_serve_get_stats.argtypes = [vp, P(MmcServeStats), vp]
+    L.mmc_serve_timestamps.argtypes = [vp, vp, u32, P(ctypes.c_uint64)]
+    L.mmc_serve_stamp.argtypes = [vp, vp]
+    L.mmc_serve_stamp_count.argtypes = [vp, P(u32), vp]
     for n in SERVE_SYMBOLS:
         getattr(L, n).restype = ctypes.c_int
     L._serve_bound = True
@@ -99,7 +103,8 @@ def expected_checksum(rid: int, length: int, token_bytes: int = 256) -> int:
 class ServeLoop:
     def __init__(self, allocator: mmc.ScatterAllocator, max_slots: int = 32768, block_bytes: int = 4096,
                  tokens_per_block: int = 16, token_bytes: int = 256, max_tokens: int = 8192,
-                 budget_bytes: int | None = None):
+                 budget_bytes: int | None = None, stamp_capacity: int = 1 << 17):
+        import torch
         L = _bind()
         self.alloc = allocator
         g = allocator.geometry()
@@ -110,7 +115,13 @@ class ServeLoop:
         self._s = ctypes.c_void_p()
         mmc._check(L.mmc_serve_create(allocator.handle, ctypes.byref(self.cfg), ctypes.byref(self._s)),
                    "mmc_serve_create")
-        self._keep = []
+        # device-clock stamp at the start of every step
+        self.d_stamps = torch.zeros(stamp_capacity, dtype=torch.int64, device="cuda")
+        hz = ctypes.c_uint64()
+        mmc._check(L.mmc_serve_timestamps(self._s, ctypes.c_void_p(self.d_stamps.data_ptr()), stamp_capacity,
+                                          ctypes.byref(hz)), "mmc_serve_timestamps")
+        self.clock_hz = hz.value
+        self._stamps = 0
 
     def close(self):
         if self._s:
@@ -139,6 +150,7 @@ class ServeLoop:
                                           ctypes.c_void_p(self.d_checksums.data_ptr()),
                                           ctypes.c_void_p(self.d_admit.data_ptr()), mmc._stream_ptr(stream)),
                    "mmc_serve_load")
+        self._stamps = 0
 
     def step(self, stream=None):
         mmc._check(_bind().mmc_serve_step(self._s, mmc._stream_ptr(stream)), "mmc_serve_step")
@@ -152,33 +164,39 @@ class ServeLoop:
     def checksums(self) -> np.ndarray:
         return self.d_checksums.cpu().numpy().view(np.uint64)[: len(self.trace.arrival)]
 
-    def run(self, steps: int, graph: bool = True, time_steps: bool = True):
-        """Runs `steps` steps; returns per-step GPU milliseconds (or None)."""
+    def run(self, steps: int, graph: bool = True, time_steps: bool = True, steps_per_graph: int = 100):
+        """Runs `steps` steps; returns each step's device time in milliseconds
+        (or None).  With `graph`, `steps_per_graph` steps are captured in one
+        graph and replayed, so only every steps_per_graph-th step pays a host
+        launch.  A step's time runs from its own device-clock stamp to the
+        next step's (the last one to a stamp enqueued after it)."""
         import torch
-        if graph:
+        k = max(1, min(steps_per_graph, steps)) if graph else 1
+        n_rep, rest = (steps // k, steps % k) if graph else (0, steps)
+        if n_rep:
             s = torch.cuda.Stream()
             s.wait_stream(torch.cuda.current_stream())
             g = torch.cuda.CUDAGraph()
             with torch.cuda.stream(s):
                 with torch.cuda.graph(g, stream=s):
-                    self.step(stream=s)
+                    for _ in range(k):
+                        self.step(stream=s)
             torch.cuda.current_stream().wait_stream(s)
             torch.cuda.synchronize()
-            launch = g.replay
-        else:
-            launch = self.step
-        events = None
-        if time_steps:
-            events = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
-            events[0].record()
-        for i in range(steps):
-            launch()
-            if time_steps:
-                events[i + 1].record()
+            for _ in range(n_rep):
+                g.replay()
+        for _ in range(rest):
+            self.step()
+        mmc._check(_bind().mmc_serve_stamp(self._s, mmc._stream_ptr()), "mmc_serve_stamp")
+        i0 = self._stamps
+        self._stamps += steps + 1
         torch.cuda.synchronize()
         if not time_steps:
             return None
-        return np.array([events[i].elapsed_time(events[i + 1]) for i in range(steps)])
+        if self._stamps > self.d_stamps.numel():
+            raise RuntimeError("stamp buffer full: raise stamp_capacity")
+        st = self.d_stamps[i0: i0 + steps + 1].cpu().numpy().astype(np.int64)
+        return np.diff(st).astype(np.float64) * 1e3 / self.clock_hz
 
 
 def run_default(log=None, duration_s: float = 30.0, check_sample: int = 64) -> dict:
@@ -200,7 +218,7 @@ def run_default(log=None, duration_s: float = 30.0, check_sample: int = 64) -> d
     parts = [loop.run(arrival_steps)]
     st = loop.stats()
     while st["completed"] < len(trace.arrival) and len(parts) < 64:
-        parts.append(loop.run(1024))  # drain the requests still decoding
+        parts.append(loop.run(1000))  # drain the requests still decoding
         st = loop.stats()
     ms = np.concatenate(parts)
     cs = loop.checksums()
@@ -216,6 +234,8 @@ def run_default(log=None, duration_s: float = 30.0, check_sample: int = 64) -> d
     out = {
         "workload": f"config 5: Poisson 2000 req/s for {duration_s:.0f} s in 10 ms steps, lognormal(6.2,0.8) tokens "
                     "<= 8192, 256 B/token, 4 KiB blocks, 32 GiB heap (one access block), pagesize 16 KiB",
+        "timing": "device clock stamp at every step start; 100 steps per graph replay (one host launch "
+                  "per 100 steps); p50/p99 over the arrival steps",
         "requests": int(len(trace.arrival)), "steps": int(len(ms)), "arrival_steps": arrival_steps,
         "p50_step_us": float(np.percentile(busy, 50) * 1e3), "p99_step_us": float(np.percentile(busy, 99) * 1e3),
         "max_step_us": float(busy.max() * 1e3),
