@@ -52,6 +52,9 @@ namespace mallocMC
             //! Claim chunks above the hierarchy threshold on several pages in
             //! one round trip (claimFlat).
             u32 flatFast = 1u;
+            //! After a window without any candidate page, skip ahead 256 pages
+            //! per round trip (fastForward).
+            u32 fastForward = 1u;
         };
 
         //! Wave-wide reductions and scans.  All 64 lanes must be active.
@@ -517,6 +520,86 @@ namespace mallocMC
                 return hw::ballot(inNeed && myRank < got);
             }
 
+            //! Skips ahead over regions without a candidate page for chunk size
+            //! c, 256 pages per round trip (each lane reads 4 pages), starting
+            //! `from` regions after region0 and stopping at the first region
+            //! that has one or at `numregions`.  Open regions passed over get
+            //! the fill-level increment an unsuccessful visit gives them
+            //! (Scatter.hpp:612-617), and the access-block cursor advances as
+            //! the walk would have (:620-629).  Returns the regions skipped.
+            //! For regionsize < 64 only.
+            template<class Wrap>
+            MALLOCMC_D u32 fastForward(u32 region0, u32 from, u32 numregions, u32 checklevel, u32 bmax, u32 maxcs,
+                                       Wrap wrap) const
+            {
+                const u32 me = hw::laneid();
+                const u32 R = h.regionsize;
+                const u32 per = umin32(64u, h.divR.div(256u)); // regions per round trip, one lane each
+                u32 skipped = 0u;
+                while(from + skipped < numregions)
+                {
+                    const u32 nregs = umin32(per, numregions - from - skipped);
+                    u32 fills[4], css[4], cnts[4];
+                    u32 regRel[4];
+#pragma unroll
+                    for(u32 l = 0; l < 4u; ++l)
+                    {
+                        const u32 pageRel = l * 64u + me;
+                        regRel[l] = h.divR.div(pageRel);
+                        fills[l] = 0xffffffffu;
+                        css[l] = 0u;
+                        cnts[l] = 0u;
+                        if(regRel[l] < nregs)
+                        {
+                            const u32 region = wrap(region0 + from + skipped + regRel[l]);
+                            const u32 page = region * R + (pageRel - regRel[l] * R);
+                            fills[l] = atom::load(&h.regions[region]);
+                            css[l] = atom::load(h.pteChunksize(page));
+                            cnts[l] = atom::load(h.pteCount(page));
+                        }
+                    }
+                    u32 first = 0xffffffffu;
+#pragma unroll
+                    for(u32 l = 0; l < 4u; ++l)
+                    {
+                        if(regRel[l] >= nregs || fills[l] >= checklevel)
+                            continue;
+                        const u32 pc = css[l];
+                        const bool cand = pc == 0u
+                            || (pc >= bmax && pc <= maxcs && pc <= h.pagesize && cnts[l] < h.pageCapacity(pc));
+                        if(cand)
+                            first = umin32(first, regRel[l]);
+                    }
+                    first = wave::min(first);
+                    const u32 passed = first == 0xffffffffu ? nregs : first;
+                    // regions passed over: fill-level increment for the open
+                    // ones (lane r holds region r's level in slice r*R/64)
+                    if(me < passed)
+                    {
+                        const u32 region = wrap(region0 + from + skipped + me);
+                        const u32 f = atom::load(&h.regions[region]);
+                        if(f < checklevel && f + 1u <= R)
+                            atom::cas(&h.regions[region], f, f + 1u);
+                    }
+                    // access-block cursor over the regions passed
+                    u32 abid = 0u;
+                    if(me < passed)
+                    {
+                        const u32 rg = wrap(region0 + from + skipped + me + 1u);
+                        const u32 q = h.divPpb.div(rg * R);
+                        if(q * h.pagesperblock == rg * R)
+                            abid = q;
+                    }
+                    abid = wave::max(abid);
+                    if(abid != 0u && me == 0u && abid > atom::load(&h.scalars->firstfreeblock))
+                        atom::store(&h.scalars->firstfreeblock, abid);
+                    skipped += passed;
+                    if(first != 0xffffffffu)
+                        break;
+                }
+                return skipped;
+            }
+
             //! Group allocation of chunk size c for the lanes in `group`.  Walks
             //! the regions in the reference's order (Scatter.hpp:563-635: from the
             //! hashed page, region by region, two passes with the 3/4 fill
@@ -566,6 +649,7 @@ namespace mallocMC
                     for(u32 done = 0; done < numregions && need != 0u; done += W)
                     {
                         const u32 nreg = umin32(W, numregions - done);
+                        bool windowHadCand = false;
                         // lane -> (region of this round, page within region)
                         const u32 ri = R >= 64u ? 0u : h.divR.div(me);
                         const u32 region = wrap(region0 + done + ri);
@@ -595,6 +679,7 @@ namespace mallocMC
                             const u64 usedAll = hw::ballot(cand && pcs != 0u);
                             const u64 freeAll = hw::ballot(cand && pcs == 0u);
                             const u64 open = hw::ballot(regionOpen);
+                            windowHadCand = windowHadCand || (usedAll | freeAll) != 0u;
                             const u32 rounds = R >= 64u ? 1u : nreg;
                             for(u32 k = 0; k < rounds && need != 0u; ++k)
                             {
@@ -660,6 +745,10 @@ namespace mallocMC
                         abid = wave::max(abid);
                         if(abid != 0u && me == 0u && abid > atom::load(&h.scalars->firstfreeblock))
                             atom::store(&h.scalars->firstfreeblock, abid);
+                        // a window with no candidate page at all: skip ahead over
+                        // the regions that have none either, 256 pages per round trip
+                        if(!windowHadCand && need != 0u && R < 64u && t.fastForward != 0u && done + W < numregions)
+                            done += fastForward(region0, done + W, numregions, checklevel, bmax, maxcs, wrap);
                     }
                     checklevel = R + 1u;
                 }

@@ -26,6 +26,8 @@ typedef struct mmc_tuning {
     uint32_t class_split;  /* size classes per octave = 2^class_split (0..3) */
     uint32_t flat_single;  /* 1: claim pages of chunks above the hierarchy threshold one at a time
                               (0, the default: several pages per round trip) */
+    uint32_t no_skip;      /* 1: walk every window of the region scan (0, the default: after a window
+                              without candidate pages, skip ahead 256 pages per round trip) */
 } mmc_tuning;
 
 int mmc_default_tuning(mmc_tuning* t);

@@ -56,24 +56,27 @@ class MmcTuning(ctypes.Structure):
         ("waves_per_cu", ctypes.c_uint32),
         ("class_split", ctypes.c_uint32),
         ("flat_single", ctypes.c_uint32),
+        ("no_skip", ctypes.c_uint32),
     ]
 
 
 @dataclasses.dataclass
 class Tuning:
     """Wave-path tuning (mmc_tuning): aggregation width 1..64, XCD slices,
-    persistent launch and its waves per CU, size classes per octave, and
-    whether flat pages are claimed one at a time (flat_single)."""
+    persistent launch and its waves per CU, size classes per octave,
+    whether flat pages are claimed one at a time (flat_single) and whether
+    the region scan walks every window (no_skip)."""
     aggregation: int = 64
     xcd_slices: int = 0
     persistent: bool = False
     waves_per_cu: int = 8
     class_split: int = 0
     flat_single: bool = False
+    no_skip: bool = False
 
     def to_c(self) -> MmcTuning:
         return MmcTuning(self.aggregation, self.xcd_slices, int(bool(self.persistent)), self.waves_per_cu,
-                         self.class_split, int(bool(self.flat_single)))
+                         self.class_split, int(bool(self.flat_single)), int(bool(self.no_skip)))
 
 
 class MmcStepDesc(ctypes.Structure):

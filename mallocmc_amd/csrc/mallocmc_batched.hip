@@ -31,6 +31,7 @@ namespace
             w.xcdSlices = t->xcd_slices;
             w.classSplit = t->class_split > 3u ? 3u : t->class_split;
             w.flatFast = t->flat_single ? 0u : 1u;
+            w.fastForward = t->no_skip ? 0u : 1u;
         }
         return w;
     }
@@ -306,6 +307,7 @@ int mmc_default_tuning(mmc_tuning* t)
     t->waves_per_cu = 8u;
     t->class_split = 0u;
     t->flat_single = 0u;
+    t->no_skip = 0u;
     return MMC_OK;
 }
 
