@@ -125,6 +125,21 @@ namespace
         finishWave(cursor, totalWaves);
     }
 
+    //! A wave-uniform copy of a struct read from memory: every 32-bit word
+    //! goes through readfirstlane, so the copy lives in scalar registers.
+    template<class T>
+    MALLOCMC_D T uniformCopy(const T& src)
+    {
+        static_assert(sizeof(T) % 4u == 0u, "word-sized struct expected");
+        T out;
+        const u32* s = reinterpret_cast<const u32*>(&src);
+        u32* o = reinterpret_cast<u32*>(&out);
+#pragma unroll
+        for(u32 i = 0; i < sizeof(T) / 4u; ++i)
+            o[i] = __builtin_amdgcn_readfirstlane(s[i]);
+        return out;
+    }
+
     MALLOCMC_D u64 mix64(u64 z)
     {
         z += 0x9E3779B97F4A7C15ull;
@@ -147,11 +162,13 @@ namespace
             if(tile >= d.n_tiles)
                 break;
             ++nTiles;
-            const u32 first = d.d_tiles[tile];
-            const u32 heapIdx = d.d_slot_heap[first];
+            const u32 first = __builtin_amdgcn_readfirstlane(d.d_tiles[tile]);
+            const u32 heapIdx = __builtin_amdgcn_readfirstlane(d.d_slot_heap[first]);
             const u32 s = first + me;
             const bool valid = s < d.n_slots && d.d_slot_heap[s] == heapIdx;
-            const ScatterHeap h = heaps[heapIdx];
+            // the view is the same for the whole wave: keep it in scalar
+            // registers instead of 50 vector registers
+            const ScatterHeap h = uniformCopy(heaps[heapIdx]);
             const WaveAlloc wa{h, t};
             const u64 r = mix64(d.seed ^ mix64(step * 0x100000001B3ull + s));
             const u32 coin = static_cast<u32>(r % 1000u);
