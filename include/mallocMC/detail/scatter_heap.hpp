@@ -276,17 +276,28 @@ namespace mallocMC
             FastDiv divSlice8; //!< pagesperblock / 8: one XCD's slice of an access block
 
             // ------------------------------------------------ addressing
+            // A PTE is 12 bytes (chunk size, count, bitmask) as in the
+            // reference.  Chunk size and count share one 8-byte-aligned word
+            // so that one 64-bit atomic reads both at the same instant: even
+            // pages hold [chunksize, count, bitmask], odd pages (whose entry
+            // starts 4 bytes past an 8-byte boundary) [bitmask, chunksize,
+            // count].  The PTE array must start 8-byte aligned.
             MALLOCMC_D u32* pteChunksize(u32 p) const
             {
-                return ptes + 3u * static_cast<u64>(p);
+                return ptes + 3u * static_cast<u64>(p) + (p & 1u);
             }
             MALLOCMC_D u32* pteCount(u32 p) const
             {
-                return ptes + 3u * static_cast<u64>(p) + 1u;
+                return ptes + 3u * static_cast<u64>(p) + 1u + (p & 1u);
             }
             MALLOCMC_D u32* pteBitmask(u32 p) const
             {
-                return ptes + 3u * static_cast<u64>(p) + 2u;
+                return ptes + 3u * static_cast<u64>(p) + ((p & 1u) != 0u ? 0u : 2u);
+            }
+            //! Chunk size (low half) and count (high half) of page p.
+            MALLOCMC_D u64* ptePair(u32 p) const
+            {
+                return reinterpret_cast<u64*>(pteChunksize(p));
             }
             MALLOCMC_D char* pageData(u32 p) const
             {

@@ -19,10 +19,12 @@
  *     atomicAnd on the segment hints and one atomicSub(count, k).
  *
  * The metadata protocol is the reference's (same PTE fields, segment masks --
- * kept in a side array here --, reset lock at count == pagesize, region fill
- * levels, access-block cursor),
- * so per-lane calls (scatter_heap.hpp) and wave calls can be mixed on one
- * heap.  Requests above a page and coalesced XMallocSIMD slices use the
+ * kept in a side array here --, region fill levels, access-block cursor;
+ * a count at or above pagesize is a reset lock and is waited out), so per-lane
+ * calls (scatter_heap.hpp) and wave calls can be mixed on one heap.  Two
+ * steps use the PTE's paired (chunk size, count) word: a reservation is one
+ * 64-bit add that also returns the chunk size, and a wave resets an empty page
+ * with one 64-bit CAS instead of the lock / drop / unlock sequence.  Requests above a page and coalesced XMallocSIMD slices use the
  * per-lane path.
  *
  * Precondition: every function here is called by all 64 lanes of the wave
@@ -209,30 +211,30 @@ namespace mallocMC
                 return L;
             }
 
-            //! One lane: resets an empty chunked page -- lock (count 0 ->
-            //! pagesize), drop its chunk size, unlock
-            //! -- if no one holds a reservation on it.  Pages of a multi-page
-            //! allocation (chunk size above the page, count 0 while they are
-            //! in use) and pagesize chunks the reference releases through the
-            //! page-based path are left alone.  Orders the unlock after the
-            //! other writes as the wave free does.
-            MALLOCMC_D void resetEmpty(u32 page) const
+            //! True for a chunk size the page reset applies to: chunked pages,
+            //! and pagesize chunks with the pagesize-leak fix.  Pages of a
+            //! multi-page allocation (chunk size above the page, count 0 while
+            //! they are in use) are left alone.
+            MALLOCMC_D bool resettable(u32 cs) const
             {
                 const u32 P = h.pagesize;
-                const u32 seen = atom::load(h.pteChunksize(page));
-                const bool chunked = seen != 0u && (seen < P || (seen == P && (h.flags & kFixPagesizeLeak)));
-                if(!chunked)
+                return cs != 0u && (cs < P || (cs == P && (h.flags & kFixPagesizeLeak)));
+            }
+
+            //! One lane: resets an empty chunked page (chunk size back to 0)
+            //! if no one holds a chunk or a reservation on it.  The reference
+            //! locks the count (0 -> pagesize), drops the chunk size and
+            //! unlocks (Scatter.hpp:686-716); chunk size and count share one
+            //! 64-bit word here, so a single CAS (cs, 0) -> (0, 0) does the
+            //! same with no lock window.  No chunk bit is set at count zero,
+            //! so no mask needs clearing.
+            MALLOCMC_D void resetEmpty(u32 page) const
+            {
+                const u64 pr = atom::load(h.ptePair(page));
+                const u32 cs = static_cast<u32>(pr);
+                if((pr >> 32) != 0u || !resettable(cs))
                     return;
-                if(atom::cas(h.pteCount(page), 0u, P) != 0u)
-                    return;
-                // under the lock the chunk size can only go from 0 to a value
-                // (a formatter or markpages); re-read it.  No chunk bit is set
-                // at count zero, so no mask needs clearing.
-                const u32 cs = atom::load(h.pteChunksize(page));
-                const bool still = cs != 0u && (cs < P || (cs == P && (h.flags & kFixPagesizeLeak)));
-                const u32 oc = still ? atom::cas(h.pteChunksize(page), cs, 0u) : 0u;
-                hw::consume(oc);
-                atom::sub(h.pteCount(page), P);
+                atom::cas(h.ptePair(page), u64{cs}, u64{0});
             }
 
             //! One lane: returns `k` reserved slots of `page`.  With
@@ -255,40 +257,48 @@ namespace mallocMC
             //!
             //! The reference's tryUsePage (Scatter.hpp:469-517) reserves with
             //! atomicAdd(count), then re-checks the chunk size with a CAS, then
-            //! reads the mask word it will claim from.  All three are issued
-            //! in one round trip here: the mask snapshot is only a hint (the
-            //! atomicOr's result decides which chunks are won), and the CAS and
-            //! the add touch different words, so their order does not matter
-            //! to any outcome the reference can reach.
+            //! reads the mask word it will claim from.  Here one 64-bit add on
+            //! the (chunk size, count) word reserves and returns the chunk size
+            //! of the same instant, so the re-check needs no second round trip;
+            //! the mask snapshot rides along (a hint only: the atomicOr's
+            //! result decides which chunks are won).  A fresh page is formatted
+            //! by a CAS issued just before the add on the same word.  The
+            //! chunk size must be read by the reservation itself: a check made
+            //! before it can be undone by a reset that completes before the
+            //! reservation lands.  When the reservation finds the page
+            //! unformatted, the claim formats it afterwards, as the reference
+            //! does.
             MALLOCMC_D u64 claimOnPage(u32 page, u32 cs, u32 bmax, u32 maxcs, u64 need, bool fresh,
                                        char*& out) const
             {
                 const u32 me = hw::laneid();
                 const u32 P = h.pagesize;
-                const bool recheck = h.resetfreedpages || fresh;
                 bool formatted = false;
                 u32 waited = 0u;
                 for(;;)
                 {
                     const u32 k = hw::popc(need);
-                    u32 old = 0u, oldcs = 0u;
+                    u32 old = 0u, rcs = 0u, fcs = 1u;
                     if(me == 0u)
                     {
-                        old = atom::add(h.pteCount(page), k);
-                        if(recheck)
-                            oldcs = atom::cas(h.pteChunksize(page), 0u, cs);
+                        if(fresh)
+                            fcs = atom::cas(h.pteChunksize(page), 0u, cs);
+                        const u64 pr = atom::add(h.ptePair(page), u64{k} << 32);
+                        old = static_cast<u32>(pr >> 32);
+                        rcs = static_cast<u32>(pr);
                     }
                     Layout L = layout(page, cs);
                     u32 snap = 0u;
                     if(me < L.nWords)
                         snap = atom::load(&L.words[me]);
                     old = wave::lane(old, 0u);
-                    oldcs = recheck ? wave::lane(oldcs, 0u) : cs;
-                    if(oldcs == 0u)
+                    rcs = wave::lane(rcs, 0u);
+                    if(wave::lane(fcs, 0u) == 0u)
                         formatted = true; // this wave (re)formatted the page
                     if(h.resetfreedpages && old >= P)
                     {
-                        // page locked by a concurrent reset
+                        // page locked by a concurrent reset of the reference
+                        // protocol (Scatter.hpp:686-716)
                         if(me == 0u)
                             backOff(page, k);
                         if(formatted && waited < kResetWaitLimit)
@@ -307,12 +317,24 @@ namespace mallocMC
                         }
                         return 0u;
                     }
-                    bool snapValid = true;
-                    if(oldcs != 0u && oldcs != cs)
+                    if(rcs == 0u)
                     {
-                        if(oldcs >= bmax && oldcs <= maxcs)
+                        // reserved on an unformatted page: format it under the
+                        // reservation (no reset can run while the count is held)
+                        u32 c2 = 0u;
+                        if(me == 0u)
+                            c2 = atom::cas(h.pteChunksize(page), 0u, cs);
+                        c2 = wave::lane(c2, 0u);
+                        if(c2 == 0u)
+                            formatted = true;
+                        rcs = c2 == 0u ? cs : c2;
+                    }
+                    bool snapValid = true;
+                    if(rcs != cs)
+                    {
+                        if(rcs >= bmax && rcs <= maxcs)
                         {
-                            cs = oldcs; // formatted meanwhile with a fitting size
+                            cs = rcs; // formatted meanwhile with a fitting size
                             L = layout(page, cs);
                             snapValid = false;
                         }
@@ -416,13 +438,14 @@ namespace mallocMC
             //! taken in the scan's preference order (used pages from the
             //! rotation point, then fresh pages) until their free chunks cover
             //! the group.  Each chosen page's lane issues, in one round trip,
-            //! the count reservation, the chunk-size check (which formats a
-            //! fresh page) and an atomicOr of the chunk bits it picked from the
-            //! scanned bitmask: ownership of a chunk is decided by the
-            //! atomicOr's result as in claimBits, so picking from a stale
-            //! bitmask only costs chunks, never hands one out twice.  A page
-            //! that turned out locked by a reset or formatted with another size
-            //! gives back what it won.  Returns the lanes served.
+            //! the format CAS of a fresh page, the 64-bit count reservation
+            //! (which returns the chunk size it was made under) and an
+            //! atomicOr of the chunk bits it picked from the scanned bitmask:
+            //! ownership of a chunk is decided by the atomicOr's result as in
+            //! claimBits, so picking from a stale bitmask only costs chunks,
+            //! never hands one out twice.  A page that turned out locked by a
+            //! reset, or whose reservation saw another chunk size, gives back
+            //! what it won.  Returns the lanes served.
             MALLOCMC_D u64 claimFlat(u32 pageL, u32 pcs, u32 pcount, u32 pbits, u64 used, u64 fresh, u32 rot,
                                      u32 c, u64 need, u64& chosen, char*& out) const
             {
@@ -462,16 +485,22 @@ namespace mallocMC
                 const u32 take = (f != 0u && pos < k) ? umin32(f, k - pos) : 0u;
                 const u32 choose = take != 0u ? wave::lowestBits(freeBits, take) : 0u;
                 chosen = hw::ballot(take != 0u);
-                // one round trip for every chosen page
-                u32 old = 0u, oldcs = 0u, oldbits = 0u;
+                // one round trip for every chosen page: a fresh page is
+                // formatted first, then the 64-bit add reserves and returns the
+                // chunk size of that instant (a reservation that finds the page
+                // unformatted gives up: the format may have landed behind it)
+                u32 old = 0u, rcs = 0u, fcs = 1u, oldbits = 0u;
                 if(take != 0u)
                 {
-                    old = atom::add(h.pteCount(pageL), take);
-                    oldcs = atom::cas(h.pteChunksize(pageL), 0u, csl);
+                    if(isFresh)
+                        fcs = atom::cas(h.pteChunksize(pageL), 0u, csl);
+                    const u64 pr = atom::add(h.ptePair(pageL), u64{take} << 32);
                     oldbits = atom::or_(h.pteBitmask(pageL), choose);
+                    old = static_cast<u32>(pr >> 32);
+                    rcs = static_cast<u32>(pr);
                 }
-                const bool formatted = take != 0u && oldcs == 0u;
-                const bool csOk = formatted || oldcs == csl;
+                const bool formatted = take != 0u && fcs == 0u;
+                const bool csOk = rcs == csl;
                 const bool locked = h.resetfreedpages && old >= P;
                 const u32 won = take != 0u ? (choose & ~oldbits) : 0u;
                 const u32 keep = (take != 0u && csOk && !locked) ? won : 0u;
@@ -982,27 +1011,16 @@ namespace mallocMC
                             atom::min(&h.scalars->firstfreeblock, block);
                     }
                 }
-                // ---- D4 for k chunks: the group that empties its page resets it
-                if(!h.resetfreedpages)
-                    return;
-                bool locked = false;
-                if(pageLeader && old == pageK)
-                    locked = atom::cas(h.pteCount(page), 0u, P) == 0u;
-                const u64 lockedLanes = hw::ballot(locked);
-                if(lockedLanes == 0u)
-                    return;
-                // order as Scatter.hpp:686-716: lock, drop the chunk size,
-                // unlock.  The reference also clears the masks (Page::init) and
-                // fences the device before the unlock.  The side masks are
-                // already clear when the count reaches zero (every free clears
-                // its bit before its count decrement), so nothing is cleared;
-                // the chunk-size CAS is a returning atomic on the PTE, and the
-                // unlock is issued once its result is back, so no L2
-                // write-back is needed.
-                const u32 oc = locked ? atom::cas(h.pteChunksize(page), cs, 0u) : 0u;
-                hw::consume(oc);
-                if(locked)
-                    atom::sub(h.pteCount(page), P);
+                // ---- D4 for k chunks: the group that empties its page resets it.
+                // The reference locks the count, clears the masks (Page::init),
+                // drops the chunk size, fences and unlocks (Scatter.hpp:686-716).
+                // The side masks hold no bit of a freed chunk (each free clears
+                // its bits; a clear still in flight only keeps a chunk that no
+                // one owns out of use until it lands), and chunk size and count
+                // share one 64-bit word, so one CAS (cs, 0) -> (0, 0) resets the
+                // page if no one reserved it meanwhile.
+                if(h.resetfreedpages && pageLeader && old == pageK)
+                    atom::cas(h.ptePair(page), u64{cs}, u64{0});
             }
         };
     } // namespace detail

@@ -90,13 +90,17 @@ int mmc_compute_geometry(const mmc_config* cfg, uint64_t bytes, mmc_geometry* ou
  * init kernel (Scatter.hpp:1057-1096) on `stream`.  Besides the pool the heap
  * holds a side array of segment masks (up to 128 B per page, at most 0.8% of
  * the pool for pagesize >= 16 KiB), where the reference keeps them in the last
- * bytes of each page; page geometry and offsets are the reference's. */
+ * bytes of each page; page geometry and offsets are the reference's.
+ * Fails with MMC_EINVAL when the page table (pool + numpages * pagesize)
+ * would not start 8-byte aligned: chunk size and count of a page are updated
+ * as one 64-bit word (possible only for a pagesize that is not a multiple
+ * of 8). */
 int mmc_create(const mmc_config* cfg, uint64_t bytes, void* stream, mmc_heap** out);
 
 /* Heap inside a pool the caller reserved (ReservePool policy,
  * reservePoolPolicies/AlpakaBuf.hpp:41-46, then AlignmentPolicy::alignPool,
- * Shrink.hpp:109-130): `pool` must already be aligned to cfg->dataAlignment.
- * Enqueues the init kernel (Scatter.hpp:1057-1096) on `stream`.  The heap does
+ * Shrink.hpp:109-130): `pool` must already be aligned to cfg->dataAlignment,
+ * and the page table must start 8-byte aligned as for mmc_create.  Enqueues the init kernel (Scatter.hpp:1057-1096) on `stream`.  The heap does
  * not own the pool: mmc_destroy leaves it to the caller. */
 int mmc_create_in_pool(const mmc_config* cfg, void* pool, uint64_t bytes, void* stream, mmc_heap** out);
 

@@ -279,6 +279,15 @@ namespace mmc
             heap->rawPool = nullptr;
             return fail(MMC_EINVAL, "heap too small: the geometry has zero pages");
         }
+        if(((reinterpret_cast<u64>(pool) + g.ptesOffset) & 7u) != 0u)
+        {
+            // chunk size and count of a PTE are updated as one 64-bit word
+            if(heap->ownsPool)
+                (void) hipFree(raw);
+            heap->rawPool = nullptr;
+            return fail(MMC_EINVAL, "the page table must start 8-byte aligned: use a pagesize that is a multiple "
+                                    "of 8 and a pool aligned to 8 bytes");
+        }
         err = hipMalloc(&heap->scalars, sizeof(ScatterScalars));
         if(err == hipSuccess)
             err = hipMalloc(&heap->scratch, sizeof(u32) * (4u * kRunThreads + 64u));

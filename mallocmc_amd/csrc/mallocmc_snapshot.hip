@@ -161,6 +161,17 @@ int mmc_snapshot(mmc_heap* heap, void* dst, uint64_t capacity, uint32_t flags, v
         err = hipStreamSynchronize(st);
     if(err != hipSuccess)
         return mmc::hipFail(err, "snapshot copy");
+    // odd pages keep [bitmask, chunksize, count] on the device (chunk size
+    // and count share an 8-byte word); the snapshot holds every PTE as
+    // [chunksize, count, bitmask]
+    for(uint64_t p = 1; p < hdr.geom.numpages; p += 2)
+    {
+        uint32_t w[3];
+        char* e = out + s.ptes + 12u * p;
+        std::memcpy(w, e, 12);
+        const uint32_t canon[3] = {w[1], w[2], w[0]};
+        std::memcpy(e, canon, 12);
+    }
     hdr.firstFreePageBased = sc.firstFreePageBased;
     hdr.firstfreeblock = sc.firstfreeblock;
     hdr.pagebasedMutex = sc.pagebasedMutex;

@@ -183,3 +183,25 @@ def test_alloc_until_null_verify_heap(mmc, cuda):
     torch.cuda.synchronize()
     assert a.getAvailableSlots(6000) == s0
     a.close()
+
+
+def test_page_table_alignment_is_checked(mmc, cuda):
+    """Chunk size and count of a PTE are updated as one 64-bit word, so the
+    page table (after numpages pages) must start 8-byte aligned: creation
+    fails with an error instead of running on a misaligned table.  Pagesize
+    1004 with regionsize 3 gives 4122 pages (aligned) at 4 MiB and 4131 pages
+    (misaligned) at 4 MiB + 10000 B."""
+    torch = cuda
+    cfg = mmc.ScatterConfig(pagesize=1004, accessblocksize=1 * MiB, regionsize=3)
+    for heap, aligned in ((4 * MiB, True), (4 * MiB + 10000, False)):
+        g = mmc.compute_geometry(cfg, heap)
+        assert ((g.numpages * 1004) % 8 == 0) == aligned
+        if aligned:
+            a = mmc.ScatterAllocator(cfg, heap)
+            offs = a.malloc_batch(torch.tensor([16, 100, 300], dtype=torch.int32, device="cuda"))
+            assert (offs.cpu().numpy() >= 0).all()
+            a.free_batch(offs)
+            a.close()
+        else:
+            with pytest.raises(mmc.MallocMCError, match="8-byte aligned"):
+                mmc.ScatterAllocator(cfg, heap)
