@@ -28,6 +28,9 @@ typedef struct mmc_tuning {
                               (0, the default: several pages per round trip) */
     uint32_t no_skip;      /* 1: walk every window of the region scan (0, the default: after a window
                               without candidate pages, skip ahead 256 pages per round trip) */
+    uint32_t no_partition; /* 1: lane i of a non-persistent launch serves request i (0, the default:
+                              each block of 256 first reorders its requests by size, its frees by
+                              page, so that a wave meets few size classes / pages) */
 } mmc_tuning;
 
 int mmc_default_tuning(mmc_tuning* t);

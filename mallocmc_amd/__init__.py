@@ -57,6 +57,7 @@ class MmcTuning(ctypes.Structure):
         ("class_split", ctypes.c_uint32),
         ("flat_single", ctypes.c_uint32),
         ("no_skip", ctypes.c_uint32),
+        ("no_partition", ctypes.c_uint32),
     ]
 
 
@@ -64,8 +65,10 @@ class MmcTuning(ctypes.Structure):
 class Tuning:
     """Wave-path tuning (mmc_tuning): aggregation width 1..64, XCD slices,
     persistent launch and its waves per CU, size classes per octave,
-    whether flat pages are claimed one at a time (flat_single) and whether
-    the region scan walks every window (no_skip)."""
+    whether flat pages are claimed one at a time (flat_single), whether
+    the region scan walks every window (no_skip) and whether a block serves
+    its requests in arrival order instead of partitioned by size
+    (no_partition)."""
     aggregation: int = 64
     xcd_slices: int = 0
     persistent: bool = False
@@ -73,10 +76,12 @@ class Tuning:
     class_split: int = 0
     flat_single: bool = False
     no_skip: bool = False
+    no_partition: bool = False
 
     def to_c(self) -> MmcTuning:
         return MmcTuning(self.aggregation, self.xcd_slices, int(bool(self.persistent)), self.waves_per_cu,
-                         self.class_split, int(bool(self.flat_single)), int(bool(self.no_skip)))
+                         self.class_split, int(bool(self.flat_single)), int(bool(self.no_skip)),
+                         int(bool(self.no_partition)))
 
 
 class MmcStepDesc(ctypes.Structure):

@@ -22,6 +22,12 @@ struct mmc_multi
 
 namespace
 {
+    //! Block partition of the one-request-per-lane kernels (on by default).
+    u32 partitionOf(const mmc_tuning* t)
+    {
+        return t && t->no_partition ? 0u : 1u;
+    }
+
     WaveTuning toTuning(const mmc_tuning* t)
     {
         WaveTuning w;
@@ -36,23 +42,106 @@ namespace
         return w;
     }
 
-    //! Lane i serves request i; the whole wave cooperates.
-    template<class Dist>
-    __global__ __launch_bounds__(256) void waveMallocKernel(ScatterHeap h, WaveTuning t, const u32* sizes,
-                                                            u64* offsets, u64 n)
+    //! Size bucket of a request for the block partition: eight buckets per
+    //! octave of (bytes - 1), so that the size classes of a wave malloc
+    //! (octaves of the request cut into up to eight parts) are unions of
+    //! buckets when the octave base is a power of two.  Larger sizes get
+    //! lower buckets: a wave serves its largest class first, and a later,
+    //! smaller request can then take a chunk of a page formatted for a larger
+    //! size (up to wastefactor times its own) instead of formatting a page of
+    //! its own.  Requests above a page and empty lanes get the last buckets.
+    constexpr u32 kSizeBuckets = 256u;
+
+    __device__ u32 sizeBucket(u32 bytes, u32 pagesize)
     {
-        const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-        const u32 bytes = i < n ? sizes[i] : 0u;
-        const WaveAlloc wa{h, t};
-        void* p = wa.malloc<Dist>(bytes);
-        if(i < n)
-            offsets[i] = h.offsetOf(p);
+        if(bytes == 0u)
+            return kSizeBuckets - 1u;
+        if(bytes > pagesize)
+            return kSizeBuckets - 2u;
+        const u32 x = bytes - 1u;
+        if(x < 8u)
+            return 223u - x;
+        const u32 e = 31u - static_cast<u32>(__builtin_clz(x)); // 3..29
+        return 223u - (8u * (e - 2u) + ((x >> (e - 3u)) & 7u));  // 0..215
     }
 
-    __global__ __launch_bounds__(256) void waveFreeKernel(ScatterHeap h, WaveTuning t, const u64* offsets, u64 n)
+    //! Reorders the 256 values of a block by an 8-bit key (LDS counting
+    //! sort, unstable).  Returns the value this thread receives and the
+    //! block slot it came from.
+    template<class T>
+    __device__ void partitionByKey(u32 key, T value, T& outValue, u32& outSlot)
+    {
+        __shared__ u32 count[kSizeBuckets];
+        __shared__ u32 start[kSizeBuckets];
+        __shared__ u32 waveTotal[4];
+        __shared__ T sortedValue[256];
+        __shared__ u32 sortedSlot[256];
+        const u32 tid = threadIdx.x;
+        count[tid] = 0u;
+        __syncthreads();
+        const u32 rank = atomicAdd(&count[key], 1u);
+        __syncthreads();
+        // exclusive scan of the 256 bucket counts: per wave, then across waves
+        const u32 c = count[tid];
+        u32 incl = c;
+        for(u32 d = 1u; d < 64u; d <<= 1)
+        {
+            const u32 up = __shfl_up(incl, d, 64);
+            if(hw::laneid() >= d)
+                incl += up;
+        }
+        if(hw::laneid() == 63u)
+            waveTotal[tid / 64u] = incl;
+        __syncthreads();
+        u32 before = 0u;
+        for(u32 w = 0; w < tid / 64u; ++w)
+            before += waveTotal[w];
+        start[tid] = before + incl - c;
+        __syncthreads();
+        const u32 pos = start[key] + rank;
+        sortedValue[pos] = value;
+        sortedSlot[pos] = tid;
+        __syncthreads();
+        outValue = sortedValue[tid];
+        outSlot = sortedSlot[tid];
+    }
+
+    //! The 256 requests of a block are served by its four waves; the whole
+    //! wave cooperates.  With `partition` the block's requests are first
+    //! reordered by size bucket (partitionByKey), and each lane serves the
+    //! request its slot received: uniformly mixed sizes put about six size
+    //! classes in a wave of 64 arrivals, and every class costs the wave one
+    //! scan and one claim.  Without it lane i serves request i.
+    template<class Dist>
+    __global__ __launch_bounds__(256) void waveMallocKernel(ScatterHeap h, WaveTuning t, const u32* sizes,
+                                                            u64* offsets, u64 n, u32 partition)
+    {
+        const u64 base = static_cast<u64>(blockIdx.x) * blockDim.x;
+        const u64 i = base + threadIdx.x;
+        u32 bytes = i < n ? sizes[i] : 0u;
+        u32 slot = threadIdx.x;
+        if(partition != 0u)
+            partitionByKey(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
+        const WaveAlloc wa{h, t};
+        void* p = wa.malloc<Dist>(bytes);
+        if(base + slot < n)
+            offsets[base + slot] = h.offsetOf(p);
+    }
+
+    //! With `partition` the block's frees are first reordered by page index
+    //! mod 256, so that the frees of one page gather in one wave and share
+    //! its atomics.
+    __global__ __launch_bounds__(256) void waveFreeKernel(ScatterHeap h, WaveTuning t, const u64* offsets, u64 n,
+                                                          u32 partition)
     {
         const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-        const u64 off = i < n ? offsets[i] : kNullOffset;
+        u64 off = i < n ? offsets[i] : kNullOffset;
+        if(partition != 0u)
+        {
+            const u32 key = off == kNullOffset ? 255u : (h.pageOf(off) & 255u);
+            u32 slot = 0u;
+            partitionByKey(key, off, off, slot);
+        }
         const WaveAlloc wa{h, t};
         wa.free(off == kNullOffset ? nullptr : h.pages + off);
     }
@@ -325,6 +414,7 @@ int mmc_default_tuning(mmc_tuning* t)
     t->class_split = 0u;
     t->flat_single = 0u;
     t->no_skip = 0u;
+    t->no_partition = 0u;
     return MMC_OK;
 }
 
@@ -356,10 +446,10 @@ int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_o
         const unsigned blocks = mmc::gridFor(n, 256);
         if(xm)
             hipLaunchKernelGGL(waveMallocKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, heap->view, t, d_sizes,
-                               d_offsets, n);
+                               d_offsets, n, partitionOf(tuning));
         else
             hipLaunchKernelGGL(waveMallocKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, heap->view, t,
-                               d_sizes, d_offsets, n);
+                               d_sizes, d_offsets, n, partitionOf(tuning));
     }
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "wave malloc launch");
@@ -378,7 +468,8 @@ int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, c
         hipLaunchKernelGGL(persistentFreeKernel, dim3(persistentBlocks(tuning)), dim3(256), 0, s, heap->view, t,
                            d_offsets, n, mmc::heapCursor(heap));
     else
-        hipLaunchKernelGGL(waveFreeKernel, dim3(mmc::gridFor(n, 256)), dim3(256), 0, s, heap->view, t, d_offsets, n);
+        hipLaunchKernelGGL(waveFreeKernel, dim3(mmc::gridFor(n, 256)), dim3(256), 0, s, heap->view, t, d_offsets, n,
+                           partitionOf(tuning));
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "wave free launch");
 }

@@ -32,6 +32,7 @@ TUNINGS = [
     dict(aggregation=16, xcd_slices=0, persistent=True),
     dict(aggregation=1, xcd_slices=0, persistent=False),
     dict(aggregation=64, xcd_slices=8, persistent=True),
+    dict(aggregation=64, xcd_slices=0, persistent=False, no_partition=True),
 ]
 
 
@@ -119,6 +120,41 @@ def test_wave_xmallocsimd_and_large(mmc, cuda):
     a.wave_free_batch(offs)
     torch.cuda.synchronize()
     assert [a.getAvailableSlots(s) for s in (16, 3 * 4096)] == s0
+    a.close()
+
+
+@pytest.mark.parametrize("no_partition", [False, True])
+def test_wave_batch_output_slots(mmc, cuda, no_partition):
+    """Each request's offset lands in its own output slot when a block
+    reorders its requests by size: checked against the heap's own metadata
+    (every live allocation is marked, fits its chunk and is not shared, and
+    the list is complete), on a batch that ends in a partial block and holds
+    empty and multi-page requests."""
+    from mallocmc_amd import snapshot
+    torch = cuda
+    t = mmc.Tuning(no_partition=no_partition)
+    cfg = mmc.ScatterConfig(pagesize=16384, accessblocksize=256 * MiB, regionsize=16, resetfreedpages=True)
+    a = mmc.ScatterAllocator(cfg, 256 * MiB)
+    g = torch.Generator().manual_seed(3)
+    n = 50000 + 77
+    sizes = torch.randint(1, 4097, (n,), generator=g)
+    sizes[torch.randint(0, n, (500,), generator=g)] = 0
+    sizes[torch.randint(0, n, (60,), generator=g)] = 3 * 16384 + 100
+    sizes = sizes.to(torch.int32).cuda()
+    print(f"wave output slots no_partition={no_partition} n={n}", flush=True)
+    offs = a.wave_malloc_batch(sizes, tuning=t)
+    o = offs.cpu().numpy()
+    z = sizes.cpu().numpy() == 0
+    assert (o[z] == -1).all()
+    assert (o[~z] >= 0).all()
+    snap = snapshot.take(a)
+    padded = ((sizes.to(torch.int64) + 15) // 16) * 16
+    r = snapshot.verify(snap, offs, padded, complete=True)
+    assert r["ok"], r["message"]
+    a.wave_free_batch(offs, tuning=t)
+    torch.cuda.synchronize()
+    r = snapshot.verify(snapshot.take(a))
+    assert r["ok"] and r["used_chunks"] == 0 and r["run_pages"] == 0, r
     a.close()
 
 
