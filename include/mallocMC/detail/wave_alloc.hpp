@@ -550,48 +550,60 @@ namespace mallocMC
             }
 
             //! Skips ahead over regions without a candidate page for chunk size
-            //! c, 256 pages per round trip (each lane reads 4 pages), starting
-            //! `from` regions after region0 and stopping at the first region
-            //! that has one or at `numregions`.  Open regions passed over get
-            //! the fill-level increment an unsuccessful visit gives them
-            //! (Scatter.hpp:612-617), and the access-block cursor advances as
-            //! the walk would have (:620-629).  Returns the regions skipped.
-            //! For regionsize < 64 only.
+            //! c, starting `from` regions after region0 and stopping at the
+            //! first region that has one or at `numregions`.  One round trip
+            //! reads the fill levels of the next 64 regions (lane r: region r)
+            //! and the PTEs of the first 256 pages (each lane reads 4 pages).
+            //! Regions filled past `checklevel` are passed over without looking
+            //! at their pages, as the walk does (Scatter.hpp:573-575), so a
+            //! stretch of such regions costs one round trip per 64 regions;
+            //! open regions are examined page by page, 256 pages per round
+            //! trip.  Open regions passed over get the fill-level increment an
+            //! unsuccessful visit gives them (Scatter.hpp:612-617; the CAS
+            //! starts from the level read after the visit: from the level read
+            //! before it, concurrent visits cancel each other's increments and
+            //! measured slower), and the access-block cursor advances as the
+            //! walk would have (:620-629).  Returns the regions skipped.  For
+            //! regionsize < 64 only.
             template<class Wrap>
             MALLOCMC_D u32 fastForward(u32 region0, u32 from, u32 numregions, u32 checklevel, u32 bmax, u32 maxcs,
                                        Wrap wrap) const
             {
                 const u32 me = hw::laneid();
                 const u32 R = h.regionsize;
-                const u32 per = umin32(64u, h.divR.div(256u)); // regions per round trip, one lane each
+                const u32 per = umin32(64u, h.divR.div(256u)); // regions whose pages one round trip reads
                 u32 skipped = 0u;
                 while(from + skipped < numregions)
                 {
-                    const u32 nregs = umin32(per, numregions - from - skipped);
-                    u32 fills[4], css[4], cnts[4];
+                    const u32 nregs = umin32(64u, numregions - from - skipped);
+                    const u32 npage = umin32(per, nregs);
+                    // lane r: fill level of region r of this round
+                    u32 fill = 0xffffffffu;
+                    if(me < nregs)
+                        fill = atom::load(&h.regions[wrap(region0 + from + skipped + me)]);
+                    u32 css[4], cnts[4];
                     u32 regRel[4];
 #pragma unroll
                     for(u32 l = 0; l < 4u; ++l)
                     {
                         const u32 pageRel = l * 64u + me;
                         regRel[l] = h.divR.div(pageRel);
-                        fills[l] = 0xffffffffu;
                         css[l] = 0u;
                         cnts[l] = 0u;
-                        if(regRel[l] < nregs)
+                        if(regRel[l] < npage)
                         {
                             const u32 region = wrap(region0 + from + skipped + regRel[l]);
                             const u32 page = region * R + (pageRel - regRel[l] * R);
-                            fills[l] = atom::load(&h.regions[region]);
                             css[l] = atom::load(h.pteChunksize(page));
                             cnts[l] = atom::load(h.pteCount(page));
                         }
                     }
+                    const u64 openMask = hw::ballot(me < nregs && fill < checklevel);
                     u32 first = 0xffffffffu;
 #pragma unroll
                     for(u32 l = 0; l < 4u; ++l)
                     {
-                        if(regRel[l] >= nregs || fills[l] >= checklevel)
+                        if(regRel[l] >= npage || ((openMask >> regRel[l]) & 1u) == 0u)
                             continue;
                         const u32 pc = css[l];
                         const bool cand = pc == 0u
@@ -600,15 +612,23 @@ namespace mallocMC
                             first = umin32(first, regRel[l]);
                     }
                     first = wave::min(first);
-                    const u32 passed = first == 0xffffffffu ? nregs : first;
-                    // regions passed over: fill-level increment for the open
-                    // ones (lane r holds region r's level in slice r*R/64)
-                    if(me < passed)
+                    u32 passed;
+                    if(first != 0xffffffffu)
+                        passed = first;
+                    else
                     {
-                        const u32 region = wrap(region0 + from + skipped + me);
-                        const u32 f = atom::load(&h.regions[region]);
+                        // past the examined regions, up to the next open one
+                        const u64 later = npage < 64u ? (openMask >> npage) << npage : 0u;
+                        passed = later != 0u ? hw::ffs(later) - 1u : nregs;
+                    }
+                    // open regions whose pages were examined without success
+                    const u32 examined = umin32(passed, npage);
+                    if(me < examined && fill < checklevel)
+                    {
+                        u32* level = &h.regions[wrap(region0 + from + skipped + me)];
+                        const u32 f = atom::load(level);
                         if(f < checklevel && f + 1u <= R)
-                            atom::cas(&h.regions[region], f, f + 1u);
+                            atom::cas(level, f, f + 1u);
                     }
                     // access-block cursor over the regions passed
                     u32 abid = 0u;
