@@ -24,8 +24,9 @@
  * calls (scatter_heap.hpp) and wave calls can be mixed on one heap.  Two
  * steps use the PTE's paired (chunk size, count) word: a reservation is one
  * 64-bit add that also returns the chunk size, and a wave resets an empty page
- * with one 64-bit CAS instead of the lock / drop / unlock sequence.  Requests above a page and coalesced XMallocSIMD slices use the
- * per-lane path.
+ * with one 64-bit CAS instead of the lock / drop / unlock sequence.
+ * Requests above a page and coalesced XMallocSIMD slices use the per-lane
+ * path.
  *
  * Precondition: every function here is called by all 64 lanes of the wave
  * in uniform control flow (lanes without work pass 0 / nullptr).
@@ -115,13 +116,12 @@ namespace mallocMC
                 return reduce(v, [](u32 a, u32 b) { return a | b; });
             }
             //! Exclusive prefix sum over lanes of values below 64, bit by bit
-            //! with ballots (scalar ops only).
+            //! with a ballot and an mbcnt per bit.
             MALLOCMC_D u32 prefix6(u32 v)
             {
-                const u64 lt = hw::lanemask_lt();
                 u32 p = 0u;
                 for(u32 b = 0; b < 6u; ++b)
-                    p += static_cast<u32>(hw::popc(hw::ballot(((v >> b) & 1u) != 0u) & lt)) << b;
+                    p += hw::rankBelow(hw::ballot(((v >> b) & 1u) != 0u)) << b;
                 return p;
             }
             //! Value of lane `l` (l uniform).
@@ -164,7 +164,7 @@ namespace mallocMC
             if(h.wastefactor < 2u)
                 return b;
             const u32 base = h.paddedMinChunk != 0u ? h.paddedMinChunk : 16u;
-            const u32 q = (b - 1u) / base;
+            const u32 q = h.pdiv(b - 1u, base);
             if(q == 0u)
                 return 0x80000000u | b; // below the octave base: exact size
             // octave j: q in [2^j, 2^(j+1)), i.e. b in (base*2^j, base*2^(j+1)]
@@ -196,7 +196,7 @@ namespace mallocMC
                 if(L.hier)
                 {
                     const u32 segmentsize = cs * 32u + 4u;
-                    L.full = umin32(32u, h.pagesize / segmentsize);
+                    L.full = umin32(32u, h.pdiv(h.pagesize, segmentsize));
                     L.additional = h.calcAdditional(L.full, segmentsize, cs);
                     L.nWords = L.full + (L.additional > 0u ? 1u : 0u);
                     L.words = h.onPageMasks(page, L.nWords);
@@ -373,7 +373,7 @@ namespace mallocMC
                 const bool hier = L.hier;
                 const u32 nWords = L.nWords, full = L.full, additional = L.additional;
                 u32* const words = L.words;
-                const u32 myRank = hw::popc(need & hw::lanemask_lt());
+                const u32 myRank = hw::rankBelow(need);
                 const bool inNeed = ((need >> me) & 1u) != 0u;
                 u64 served = 0u;
                 u32 remaining = grant;
@@ -385,7 +385,7 @@ namespace mallocMC
                     u32 spots = 0u, snap = 0u;
                     if(owner)
                     {
-                        spots = hier ? (me < full ? 32u : additional) : umin32(P / cs, 32u);
+                        spots = hier ? (me < full ? 32u : additional) : umin32(h.pdiv(P, cs), 32u);
                         snap = (pass == 0u && haveSnap) ? snap0 : atom::load(&words[me]);
                     }
                     const u32 sel = spots == 32u ? ~0u : ((1u << spots) - 1u);
@@ -454,7 +454,7 @@ namespace mallocMC
                 const bool isUsed = ((used >> me) & 1u) != 0u;
                 const bool isFresh = ((fresh >> me) & 1u) != 0u;
                 const u32 csl = isUsed ? pcs : c;
-                const u32 capl = (isUsed || isFresh) ? umin32(P / csl, 32u) : 0u;
+                const u32 capl = (isUsed || isFresh) ? umin32(h.pdiv(P, csl), 32u) : 0u;
                 const u32 selBits = capl >= 32u ? ~0u : ((1u << capl) - 1u);
                 const u32 freeBits = (isUsed || isFresh) ? (~pbits & selBits) : 0u;
                 const u32 room = isUsed ? (capl > pcount ? capl - pcount : 0u) : capl;
@@ -530,7 +530,7 @@ namespace mallocMC
                         atom::sub(h.pteCount(pageL), take - kept); // holds kept chunks: count stays > 0
                 }
                 // hand the kept chunks to the group's lanes in rank order
-                const u32 myRank = static_cast<u32>(hw::popc(need & hw::lanemask_lt()));
+                const u32 myRank = hw::rankBelow(need);
                 const bool inNeed = ((need >> me) & 1u) != 0u;
                 u32 got = 0u;
                 for(u64 pagesKept = hw::ballot(kept != 0u); pagesKept != 0u; pagesKept &= pagesKept - 1u)
@@ -1005,7 +1005,7 @@ namespace mallocMC
                 if(segLeader)
                 {
                     const u32 segmentsize = cs * 32u + 4u;
-                    const u32 full = umin32(32u, P / segmentsize);
+                    const u32 full = umin32(32u, h.pdiv(P, segmentsize));
                     const u32 additional = h.calcAdditional(full, segmentsize, cs);
                     u32* masks = h.onPageMasks(page, full + (additional > 0u ? 1u : 0u));
                     atom::and_(&masks[seg], ~segBits);
@@ -1020,7 +1020,7 @@ namespace mallocMC
                     // threshold is 0 and never fires, so failed scans ratchet every
                     // region to "full" and each later request scans the whole heap.
                     // Test first so hot words are only written when they change.
-                    const u32 thr = umax32(P / 2u / cs, 1u);
+                    const u32 thr = umax32(h.pdiv(P / 2u, cs), 1u);
                     if(old >= thr && old - pageK < thr)
                     {
                         u32* region = &h.regions[h.divR.div(page)];

@@ -148,6 +148,16 @@ namespace mallocMC
             return static_cast<std::uint32_t>(__popcll(static_cast<unsigned long long>(m)));
         }
 
+        //! Set bits of m below this lane, popc(m & ((1 << lane) - 1)): two
+        //! mbcnt instructions.  Equals popc(m & lanemask_lt()) when m holds
+        //! bits of active lanes only (a ballot or a subset of one).
+        MALLOCMC_D std::uint32_t rankBelow(std::uint64_t m)
+        {
+            return __builtin_amdgcn_mbcnt_hi(
+                static_cast<std::uint32_t>(m >> 32),
+                __builtin_amdgcn_mbcnt_lo(static_cast<std::uint32_t>(m), 0u));
+        }
+
         template<class T>
         MALLOCMC_D T readlane(T v, std::uint32_t lane)
         {

@@ -69,6 +69,31 @@ def test_serial_random_streams(mmc, oracle_mod, cuda, seed, pagesize, reset, reg
     assert (want == -1).any() or True  # streams may or may not reach OOM
 
 
+@pytest.mark.parametrize("seed,pagesize,wastefactor,align,region", [
+    (11, 4096, 1, 16, 16), (12, 16384, 3, 16, 8), (13, 65536, 2, 64, 16), (14, 8192, 4, 32, 4),
+    (15, 4096, 2, 8, 8),
+])
+def test_serial_threshold_sizes(mmc, oracle_mod, cuda, seed, pagesize, wastefactor, align, region):
+    """Sizes at the policy boundaries: around the hierarchy threshold
+    (pagesize-8)/33, the minimum chunk (pagesize-128)/1024, the XMallocSIMD
+    bound pagesize/64, half a page and a page, under several wastefactors and
+    data alignments, with interleaved frees on a heap small enough to fill."""
+    rng = np.random.default_rng(seed)
+    ht = (pagesize - 8) // 33
+    mc = (pagesize - 128) // 1024
+    cand = sorted({x for x in [1, mc - 1, mc, mc + 1, ht - align, ht - 1, ht, ht + 1, ht + align, 2 * ht,
+                               pagesize // 64 - 1, pagesize // 64, pagesize // 2 - 1, pagesize // 2,
+                               pagesize // 2 + 1, pagesize - 1, pagesize, pagesize + 1] if x > 0})
+    sizes = rng.choice(np.array(cand), size=5000)
+    ops = wl.random_stream(rng, 4000, sizes, p_free=0.3)
+    heap = max(6 * pagesize * region * 4, 2 * MiB)
+    cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=heap // 2, regionsize=region, wastefactor=wastefactor,
+                            resetfreedpages=True, dataAlignment=align)
+    want, got, ws, gs = _serial_pair(mmc, oracle_mod, cuda, cfg, heap, ops)
+    np.testing.assert_array_equal(got, want)
+    assert gs == ws
+
+
 def test_serial_fill_to_oom_and_refill(mmc, oracle_mod, cuda):
     """Fill a tiny heap until null, free every other allocation, refill."""
     cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=256 * 1024, regionsize=4, resetfreedpages=True)
