@@ -81,8 +81,9 @@ int mmc_serve_load(mmc_serve* s, const uint32_t* d_arrival, const uint32_t* d_le
                    uint32_t n, uint64_t* d_checksums, uint32_t* d_admit_step, void* stream);
 
 /* Enqueues one step (decode, then completion with the next admission) on
- * `stream`.  With a timestamp buffer attached, the step first records the
- * device clock into the next entry of that buffer. */
+ * `stream`.  With a timestamp buffer attached, the decode kernel's first
+ * workgroup records the device clock into the next entry of that buffer
+ * before any other work of the step. */
 int mmc_serve_step(mmc_serve* s, void* stream);
 
 /* Attaches a device buffer of `capacity` uint64 device-clock stamps (NULL

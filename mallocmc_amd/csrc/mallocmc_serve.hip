@@ -62,6 +62,8 @@ namespace
         u32 tokenBytes;
         u32 maxBlocks;
         u64 budget;
+        u64* stamps; // device-clock stamp per step (null: none)
+        u32 stampCapacity;
     };
 
     __device__ __forceinline__ u32 payloadWord(u32 r, u32 t, u32 w)
@@ -157,8 +159,20 @@ namespace
     }
 
     //! Step part 1, one lane per request slot: append one token.
+    //! Appends the device clock to the stamp buffer (one lane).
+    __device__ void stamp(ServeScalars* sc, u64* stamps, u32 capacity)
+    {
+        const u64 t = wall_clock64();
+        const u32 i = atom::add(&sc->nStamps, 1u);
+        if(i < capacity)
+            stamps[i] = t;
+    }
+
     __global__ __launch_bounds__(256) void decodeKernel(ServeView v)
     {
+        // the step's start stamp: the first workgroup, before any work
+        if(v.stamps && blockIdx.x == 0u && threadIdx.x == 0u)
+            stamp(v.sc, v.stamps, v.stampCapacity);
         const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
         const bool valid = slot < v.maxSlots;
         const u32 req = valid ? v.slotReq[slot] : kEmpty;
@@ -372,15 +386,11 @@ namespace
         }
     }
 
-    //! Appends the device clock to the stamp buffer (one lane).
+    //! A stamp outside a step (e.g. after the last one).
     __global__ void stampKernel(ServeScalars* sc, u64* stamps, u32 capacity)
     {
-        if(blockIdx.x != 0u || threadIdx.x != 0u)
-            return;
-        const u64 t = wall_clock64();
-        const u32 i = atom::add(&sc->nStamps, 1u);
-        if(i < capacity)
-            stamps[i] = t;
+        if(blockIdx.x == 0u && threadIdx.x == 0u)
+            stamp(sc, stamps, capacity);
     }
 
     //! Admission for step 0 after the reset (one wave).
@@ -484,8 +494,6 @@ int mmc_serve_step(mmc_serve* s, void* stream)
     if(!s)
         return mmc::fail(MMC_EINVAL, "null argument");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if(s->stamps)
-        hipLaunchKernelGGL(stampKernel, dim3(1), dim3(64), 0, st, s->view.sc, s->stamps, s->stampCapacity);
     hipLaunchKernelGGL(decodeKernel, dim3(mmc::gridFor(s->cfg.max_slots, 256)), dim3(256), 0, st, s->view);
     hipLaunchKernelGGL(completeKernel, dim3(kCompleteBlocks), dim3(256), 0, st, s->view);
     const hipError_t err = hipGetLastError();
@@ -498,6 +506,8 @@ int mmc_serve_timestamps(mmc_serve* s, uint64_t* d_stamps, uint32_t capacity, ui
         return mmc::fail(MMC_EINVAL, "null argument");
     s->stamps = capacity ? d_stamps : nullptr;
     s->stampCapacity = d_stamps ? capacity : 0u;
+    s->view.stamps = s->stamps;
+    s->view.stampCapacity = s->stampCapacity;
     if(clock_hz)
     {
         int khz = 0;
