@@ -175,26 +175,41 @@ namespace
             stamp(v.sc, v.stamps, v.stampCapacity);
         const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
         const bool valid = slot < v.maxSlots;
-        const u32 req = valid ? v.slotReq[slot] : kEmpty;
+        // the slot's state in one round trip (a free slot's position and
+        // length are not used), then the request id and the current block
+        // in the next, issued before the malloc's own loads
+        u32 req = kEmpty, t = 0u, len = 0u;
+        if(valid)
+        {
+            req = v.slotReq[slot];
+            t = v.slotCur[slot];
+            len = v.slotLen[slot];
+        }
         const bool active = req != kEmpty;
-        const u32 t = active ? v.slotCur[slot] : 0u;
         const bool needBlock = active && (t % v.tokensPerBlock) == 0u;
+        u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
+        u32 id = 0u;
+        u64 off = kNullOffset;
+        if(active)
+        {
+            id = v.ids[req];
+            if(!needBlock)
+                off = blocks[t / v.tokensPerBlock];
+        }
         const WaveAlloc wa{v.heap, WaveTuning{}};
         void* p = wa.malloc<NoopDistribution>(needBlock ? v.blockBytes : 0u);
-        u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
         u32 nulls = 0u;
         if(needBlock)
         {
-            blocks[t / v.tokensPerBlock] = v.heap.offsetOf(p);
+            off = v.heap.offsetOf(p);
+            blocks[t / v.tokensPerBlock] = off;
             if(p == nullptr)
                 nulls = 1u;
         }
         if(active)
         {
-            const u64 off = blocks[t / v.tokensPerBlock];
             if(off != kNullOffset)
             {
-                const u32 id = v.ids[req];
                 u32* dst = reinterpret_cast<u32*>(v.heap.pages + off
                                                   + static_cast<u64>(t % v.tokensPerBlock) * v.tokenBytes);
                 const u32 words = v.tokenBytes / 4u;
@@ -210,7 +225,7 @@ namespace
             }
             const u32 t1 = t + 1u;
             v.slotCur[slot] = t1;
-            if(t1 == v.slotLen[slot])
+            if(t1 == len)
                 v.doneList[atom::add(&v.sc->nDone, 1u)] = slot;
         }
         u32 mallocs = needBlock ? 1u : 0u;
@@ -234,11 +249,11 @@ namespace
     //! Returns a finished request's blocks (wave-cooperative free), slot and
     //! reservation; the whole wave calls it.  The results that the admission
     //! at the end of the launch reads are returning atomics, waited for here.
-    __device__ void release(const ServeView& v, u32 slot)
+    __device__ void release(const ServeView& v, u32 slot, u32 len)
     {
         const u32 lane = hw::laneid();
         const WaveAlloc wa{v.heap, WaveTuning{}};
-        const u32 nb = divup(atom::load(&v.slotLen[slot]), v.tokensPerBlock);
+        const u32 nb = divup(len, v.tokensPerBlock);
         const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
         u32 freed = 0u;
         for(u32 b = 0; b < nb; b += 64u)
@@ -266,8 +281,8 @@ namespace
     //! Step part 2, completion: the paged buffers of the finished requests,
     //! cut into 64-token chunks spread over the whole grid, are read back into
     //! their checksums (lane = word of a token, the 4 waves of a workgroup take
-    //! every fourth token, 4 tokens in flight per lane; partial sums add
-    //! atomically).  The workgroup that finishes a request's last chunk
+    //! every fourth token, all 16 of a wave's tokens in flight at once;
+    //! partial sums add atomically).  The workgroup that finishes a request's last chunk
     //! releases the request; the last workgroup to finish resets the done list
     //! and admits the next step's arrivals, so one step is two launches.
     __global__ __launch_bounds__(256) void completeKernel(ServeView v)
@@ -281,53 +296,71 @@ namespace
         u32 i = 0u, base = 0u; // done-list entry and its first chunk id
         for(u32 c = blockIdx.x;; c += gridDim.x)
         {
-            u32 nChunks = 0u;
+            // the entry holding chunk c: 64 entries per round trip (lane k
+            // reads entry i + k and its length), a prefix sum of their chunk
+            // counts and a ballot, instead of one dependent load pair per entry
+            u32 nChunks = 0u, slot = 0u, len = 0u;
+            bool found = false;
             while(i < nDone)
             {
-                nChunks = divup(v.slotLen[v.doneList[i]], kTokensPerChunk);
-                if(c < base + nChunks)
+                const u32 e = i + lane;
+                const u32 es = e < nDone ? v.doneList[e] : 0u;
+                const u32 el = e < nDone ? v.slotLen[es] : 0u;
+                const u32 n = divup(el, kTokensPerChunk);
+                u32 incl = n;
+                for(u32 d = 1u; d < 64u; d <<= 1)
+                {
+                    const u32 up = __shfl_up(incl, d, 64);
+                    if(lane >= d)
+                        incl += up;
+                }
+                const u64 past = hw::ballot(c < base + incl); // entries whose range ends after c
+                if(past != 0u)
+                {
+                    const u32 k = hw::ffs(past) - 1u;
+                    nChunks = wave::lane(n, k);
+                    slot = wave::lane(es, k);
+                    len = wave::lane(el, k);
+                    base += wave::lane(incl, k) - nChunks;
+                    i += k;
+                    found = true;
                     break;
-                base += nChunks;
-                ++i;
+                }
+                base += wave::lane(incl, 63u);
+                i += 64u;
             }
-            if(i >= nDone)
+            if(!found)
                 break;
-            const u32 slot = v.doneList[i];
             const u32 req = v.slotReq[slot];
-            const u32 len = v.slotLen[slot];
             const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
             const u32 t0 = (c - base) * kTokensPerChunk;
             const u32 t1 = umin32(len, t0 + kTokensPerChunk);
+            // wave q reads tokens t0 + q, t0 + q + 4, ...: all block offsets
+            // of its tokens in one round trip, then all payload words in the next
+            constexpr u32 kPerWave = kTokensPerChunk / 4u;
+            u64 offs[kPerWave];
+#pragma unroll
+            for(u32 j = 0; j < kPerWave; ++j)
+            {
+                const u32 tt = t0 + q + 4u * j;
+                offs[j] = tt < t1 ? blocks[tt / v.tokensPerBlock] : kNullOffset;
+            }
             u64 cs = 0u;
             for(u32 w = lane; w < wpt; w += 64u)
             {
-                u32 t = t0 + q;
-                for(; t + 12u < t1; t += 16u)
-                {
-                    u32 x[4];
+                u32 x[kPerWave];
 #pragma unroll
-                    for(u32 j = 0; j < 4u; ++j)
-                    {
-                        const u32 tt = t + 4u * j;
-                        const u64 off = blocks[tt / v.tokensPerBlock];
-                        x[j] = off == kNullOffset
-                            ? 0u
-                            : *reinterpret_cast<const u32*>(
-                                v.heap.pages + off + static_cast<u64>(tt % v.tokensPerBlock) * v.tokenBytes + 4u * w);
-                    }
-#pragma unroll
-                    for(u32 j = 0; j < 4u; ++j)
-                        cs += static_cast<u64>(x[j]) * (2u * (static_cast<u64>(t + 4u * j) * wpt + w) + 1u);
-                }
-                for(; t < t1; t += 4u)
+                for(u32 j = 0; j < kPerWave; ++j)
                 {
-                    const u64 off = blocks[t / v.tokensPerBlock];
-                    if(off == kNullOffset)
-                        continue;
-                    const u32 x = *reinterpret_cast<const u32*>(
-                        v.heap.pages + off + static_cast<u64>(t % v.tokensPerBlock) * v.tokenBytes + 4u * w);
-                    cs += static_cast<u64>(x) * (2u * (static_cast<u64>(t) * wpt + w) + 1u);
+                    const u32 tt = t0 + q + 4u * j;
+                    x[j] = offs[j] == kNullOffset
+                        ? 0u
+                        : *reinterpret_cast<const u32*>(
+                            v.heap.pages + offs[j] + static_cast<u64>(tt % v.tokensPerBlock) * v.tokenBytes + 4u * w);
                 }
+#pragma unroll
+                for(u32 j = 0; j < kPerWave; ++j)
+                    cs += static_cast<u64>(x[j]) * (2u * (static_cast<u64>(t0 + q + 4u * j) * wpt + w) + 1u);
             }
             for(int d = 32; d > 0; d >>= 1)
                 cs += __shfl_xor(cs, d, 64);
@@ -343,7 +376,7 @@ namespace
             }
             __syncthreads();
             if(lastChunk && q == 0u)
-                release(v, slot);
+                release(v, slot, len);
             __syncthreads();
         }
         __shared__ u32 last;
