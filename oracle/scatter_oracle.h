@@ -12,6 +12,24 @@
  * benchmark may load this library, and only as the checker or the reported
  * CPU baseline.  The product (libmallocmc_hip.so) never links or calls it.
  *
+ * What pins this restatement (tests/test_oracle.py): closed forms of the
+ * geometry, the page capacities and the first serialized offset of every
+ * size on a fresh heap; the slot counts and first offset a probe build of the
+ * reference printed for config 1; brute-force streams on tiny heaps checking
+ * after every step that no allocations overlap, the wastefactor rule holds, a
+ * page hands out its lowest free chunk, the count equals the live chunks and a
+ * null means no page can take the request; the special cases of 0 bytes,
+ * multi-page first fit from the top and the pagesize-chunk leak.
+ *
+ * Parity unpinned: the order in which the region walk visits pages once a
+ * size has pages in use -- the region fill-level increments and resets
+ * (Scatter.hpp:612-617, 727-735), the access-block cursor (:620-629,
+ * 733-734), _firstFreePageBased (:796-801, 885) and the switch to the second
+ * pass (:553, 634).  These follow the reference by code reading only: the
+ * reference holds no vectors for them and its CPU build needs Boost, which is
+ * absent.  The GPU path matches this oracle on them bit-exactly, but no check
+ * independent of the oracle fixes them.
+ *
  * Documented divergences from the reference (all cases where the reference is
  * undefined or unsafe; none is reachable by a valid workload otherwise):
  *   D1 heap < accessblocksize: the reference divides by zero
