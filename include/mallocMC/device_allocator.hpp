@@ -32,7 +32,14 @@ namespace mallocMC
 
         //! DeviceAllocator::malloc (device_allocator.hpp:72-82).  Callable from
         //! any lane; with XMallocSIMD the lanes of a wavefront that call it
-        //! together share one allocation.
+        //! together share one allocation.  When all 64 lanes of the wavefront
+        //! call it together (full active mask) and none asks for more than a
+        //! page, they are served by the wave-cooperative path (malloc_wave):
+        //! same heap protocol, one group claim per size class instead of one
+        //! reference walk per lane.  Otherwise each lane runs the reference's
+        //! per-lane algorithm (Scatter.hpp:895-912 and what it calls); its
+        //! page-based path serializes runs of pages on one mutex and keeps
+        //! _firstFreePageBased exact, so runs stay one short scan each.
         template<typename Acc>
         __device__ void* malloc(const Acc&, std::size_t bytes) const
         {
@@ -40,6 +47,14 @@ namespace mallocMC
         }
 
         __device__ void* malloc(std::size_t bytes) const
+        {
+            if(hw::activemask() == ~std::uint64_t{0} && hw::ballot(bytes > heap.pagesize) == 0u)
+                return malloc_wave(bytes);
+            return mallocLane(bytes);
+        }
+
+        //! The reference's per-lane malloc, whatever the active mask.
+        __device__ void* mallocLane(std::size_t bytes) const
         {
             const std::uint32_t padded = AlignmentPolicy::applyPadding(static_cast<std::uint32_t>(bytes));
             typename DistributionPolicy::Device dist;
@@ -57,7 +72,18 @@ namespace mallocMC
             free(p);
         }
 
+        //! With a full active mask the wavefront frees cooperatively
+        //! (free_wave: frees grouped by page); otherwise per lane.
         __device__ void free(void* p) const
+        {
+            if(hw::activemask() == ~std::uint64_t{0})
+                free_wave(p);
+            else
+                freeLane(p);
+        }
+
+        //! The reference's per-lane free (Scatter.hpp:918-946).
+        __device__ void freeLane(void* p) const
         {
             heap.destroy<detail::WaveIds>(p);
         }

@@ -36,3 +36,11 @@ def test_example_slots():
 def test_verify_heap(heap_mb, blocks, threads):
     r = _run("verify_heap", str(heap_mb), str(blocks), str(threads))
     assert r.returncode == 0 and "verify_heap OK" in r.stdout
+
+
+def test_thread_malloc_drop_in():
+    """1M threads malloc and free through the drop-in handle: full
+    wavefronts take the wave-cooperative path, wavefronts with a lane missing
+    the per-lane path; both keep the blocks intact and restore the slots."""
+    r = _run("thread_malloc", timeout=150)
+    assert r.returncode == 0 and "thread_malloc OK" in r.stdout
