@@ -835,6 +835,10 @@ namespace mallocMC
                 {
                     long long top = sweep == 0u ? static_cast<long long>(start) : static_cast<long long>(numpages) - 1;
                     const long long bottom = sweep == 0u ? 0 : static_cast<long long>(start);
+                    // no two adjacent free pages seen between the cursor and the
+                    // windows so far (a run needs two pages or more, so a single
+                    // free page between runs cannot serve one)
+                    bool usedAbove = sweep == 0u;
                     while(top >= bottom)
                     {
                         const long long base = top - 63 > bottom ? top - 63 : bottom;
@@ -843,6 +847,7 @@ namespace mallocMC
                         if(pg <= top)
                             isFree = atom::load(h.pteChunksize(static_cast<u32>(pg))) == 0u;
                         const u64 m = hw::ballot(isFree);
+                        const u64 pairs = m & (m >> 1); // bit i: pages i and i+1 free
                         u64 r = m;
                         for(u32 j = 1; j < k; ++j)
                             r &= m >> j;
@@ -856,12 +861,23 @@ namespace mallocMC
                             ok = wave::lane(ok, 0u);
                             if(ok)
                             {
-                                if(me == 0u && sp + k - 1u == start && sp > 0u)
-                                    atom::cas(&h.scalars->firstFreePageBased, start, sp - 1u);
+                                // no run could start between the cursor and this
+                                // run: move the cursor below the run, as the
+                                // reference does when every page above was in use
+                                // (Scatter.hpp:791-798).  Runs claimed concurrently
+                                // leave the cursor behind the pages they took, so
+                                // lower it with a min instead of the reference's
+                                // CAS from the value read.
+                                const u64 above = pos + k < 64u ? (pairs >> (pos + k)) : 0u;
+                                if(me == 0u && usedAbove && above == 0u && sp > 0u)
+                                    atom::min(&h.scalars->firstFreePageBased, sp - 1u);
                                 return h.pageData(sp);
                             }
                             r &= ~(1ull << pos);
                         }
+                        // free pairs of this window lie above any run found lower
+                        if(pairs != 0u)
+                            usedAbove = false;
                         if(base == bottom)
                             break;
                         top = base + static_cast<long long>(k) - 2; // windows overlap by k-1 pages
@@ -886,11 +902,23 @@ namespace mallocMC
                 const u32 b = dist.collect(h, h.pad(bytesIn));
                 char* result = nullptr;
                 bool want = b != 0u;
-                // requests above a page: one wave-wide run claim each
+                // requests above a page: one wave-wide run claim each.  Once a
+                // run of k pages was not found anywhere, the wave's later
+                // requests of k or more pages get null without another search
+                // (as if served at the same instant).
+                u32 failedPages = 0xffffffffu;
                 for(u64 big = hw::ballot(want && b > h.pagesize); big != 0u; big &= big - 1u)
                 {
                     const u32 l = hw::ffs(big) - 1u;
-                    char* r = allocRun(wave::lane(b, l));
+                    const u32 bl = wave::lane(b, l);
+                    const u32 kl = h.divP.div(bl + h.pagesize - 1u);
+                    char* r = nullptr;
+                    if(kl < failedPages)
+                    {
+                        r = allocRun(bl);
+                        if(r == nullptr && kl <= 32u)
+                            failedPages = kl;
+                    }
                     if(me == l)
                         result = r;
                 }

@@ -158,6 +158,47 @@ def test_wave_batch_output_slots(mmc, cuda, no_partition):
     a.close()
 
 
+def test_wave_multipage_fill_to_null(mmc, cuda):
+    """Two-page requests until the heap is full, through the wave path: every
+    wave claims up to 64 runs, the page-based cursor follows the claimed runs
+    down, and a wave whose run search failed hands null to its later
+    requests.  Nulls come only once no two adjacent pages are free, the
+    heap's metadata matches the live runs, and free-all restores the slots."""
+    from mallocmc_amd import snapshot
+    torch = cuda
+    cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=32 * MiB, regionsize=16, resetfreedpages=True)
+    a = mmc.ScatterAllocator(cfg, 32 * MiB)
+    g = a.geometry()
+    s0 = a.getAvailableSlots(6000)
+    n = int(g.numpages) // 2 + 2000
+    sizes = torch.full((n,), 6000, dtype=torch.int32, device="cuda")
+    print(f"multipage fill: {g.numpages} pages, {n} two-page requests", flush=True)
+    offs = a.wave_malloc_batch(sizes)
+    served = int((offs >= 0).sum().item())
+    snap = snapshot.take(a)
+    hdr = snapshot.header(snap)
+    pte = np.frombuffer(bytes(snap[hdr.off_ptes: hdr.off_ptes + 12 * int(g.numpages)]), dtype=np.uint32)
+    free = pte.reshape(-1, 3)[:, 0] == 0
+    free_pairs = int((free[:-1] & free[1:]).sum())
+    print(f"  served {served}, free pages {int(free.sum())}, free pairs {free_pairs}, "
+          f"page-based cursor {hdr.firstFreePageBased}", flush=True)
+    # a null means no two adjacent free pages were left (concurrent claims
+    # leave single pages between runs, which no run can use)
+    assert served < n and free_pairs == 0
+    assert served >= 0.75 * (int(g.numpages) // 2)
+    a.write_pattern(offs, sizes, salt=9)
+    r = a.verify_pattern(offs, sizes, salt=9)
+    assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0, r
+    assert no_overlap(torch, offs, sizes)
+    padded = ((sizes.to(torch.int64) + 15) // 16) * 16
+    rep = snapshot.verify(snap, offs, padded, complete=True)
+    assert rep["ok"], rep["message"]
+    a.wave_free_batch(offs)
+    torch.cuda.synchronize()
+    assert a.getAvailableSlots(6000) == s0
+    a.close()
+
+
 def _step_setup(torch, mmc, nheaps, slots_per_heap, heap_bytes, pagesize, seed=7, nsizes=8):
     cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=heap_bytes // 4, regionsize=16,
                             resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
