@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Headline benchmark: mallocs+frees/s at 1,048,576 lanes, batched vs plain, one MI355X.
 
-Workload (config 2 of the benchmark set): heap 4 GiB, 1,048,576 requests per
+Workload (the mixed-size batch): heap 4 GiB, 1,048,576 requests per
 step with sizes 16 * (1 + splitmix64(seed << 32 | lane) mod 256) B (uniform
 16..4096 B), seeds 1, 2, 3 in turn; a step is one malloc launch over all
 requests followed by one free launch over the results.  The size arrays sit
@@ -17,9 +17,9 @@ Forms:
 Also reported: the device-atomic ceiling (microbenchmark), the roofline of
 the dominant kernel, the CPU baseline (the oracle on this host's cores, the
 reference's CpuOmp2Blocks execution model with 16-way mutex atomics), the
-serving-loop p50/p99 step latency (config 5, mallocmc_amd/serve.py), the
-multi-heap batched form (config 3, mallocmc_amd/multiheap.py) and the
-fragmentation/OOM stress cycles (config 4, mallocmc_amd/oomstress.py).
+serving-loop p50/p99 step latency (mallocmc_amd/serve.py), the
+multi-heap batched form (mallocmc_amd/multiheap.py) and the
+fragmentation/OOM stress cycles (mallocmc_amd/oomstress.py).
 """
 from __future__ import annotations
 
@@ -114,7 +114,7 @@ def cpu_baseline(sizes_np):
     h.close()
     ops = 2 * n / (res.t_malloc + res.t_free)
     return {"value": ops, "unit": "mallocs+frees/s", "cores": res.threads_used, "kind": "port",
-            "sample": f"config 2 seed 1: {n} requests, 4 GiB heap, pagesize 16 KiB, one malloc and one free "
+            "sample": f"mixed-size batch, seed 1: {n} requests, 4 GiB heap, pagesize 16 KiB, one malloc and one free "
                       f"each, blocks x 1 thread over OpenMP (CpuOmp2Blocks), 16-way mutex atomics",
             "nulls": int(res.nulls), "mismatches": int(res.mismatches),
             "t_malloc_s": res.t_malloc, "t_free_s": res.t_free}
@@ -130,8 +130,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="use the stored best point instead of tuning")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-multi", action="store_true", help="skip the multi-heap (config 3) section")
-    ap.add_argument("--no-oom", action="store_true", help="skip the OOM-stress (config 4) section")
+    ap.add_argument("--no-multi", action="store_true", help="skip the multi-heap section")
+    ap.add_argument("--no-oom", action="store_true", help="skip the OOM-stress section")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -233,7 +233,7 @@ def main():
     mal_avg_ms = float(np.mean(mal_ms[1:]))
     # HBM bytes per malloc launch from the committed counter passes (see the file)
     traffic = None
-    tpath = os.path.join(HERE, "profiles", "batched_config2_hbm_traffic.json")
+    tpath = os.path.join(HERE, "profiles", "batched_mixed1M_hbm_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("waveMallocKernel<NoopDistribution>", {}).get("hbm_bytes_per_launch")
@@ -268,12 +268,12 @@ def main():
     multi = None
     if not args.no_multi:
         from mallocmc_amd import multiheap
-        multi = multiheap.run_config3(log=log, tune_steps=10)
+        multi = multiheap.run_default(log=log, tune_steps=10)
 
     oom = None
     if not args.no_oom:
         from mallocmc_amd import oomstress
-        oom = oomstress.run_config4(log=log)
+        oom = oomstress.run_default(log=log)
 
     serve = None
     if not args.no_serve:
@@ -303,7 +303,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": "config 2: 1,048,576 requests/step, sizes 16..4096 B (splitmix64 seeds 1,2,3), 4 GiB heap",
+            "workload": "mixed-size batch: 1,048,576 requests/step, sizes 16..4096 B (splitmix64 seeds 1,2,3), 4 GiB heap",
             "form": "batched+autotuned (wave-cooperative, hipGraph malloc+free per step)",
             "pagesize": best.pagesize, "regionsize": best.regionsize, "aggregation": best.aggregation,
             "persistent": best.persistent, "xcd_slices": best.xcd_slices, "resetfreedpages": True,

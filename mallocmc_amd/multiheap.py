@@ -1,4 +1,4 @@
-"""Config 3: many independent heaps served by one persistent launch per step.
+"""Many independent heaps served by one persistent launch per step.
 
 64 instances; instance i gets floor(1024 * 256**u_i) request slots (u_i
 uniform, so log-uniform in [1024, 262144]) and its own 256 MiB heap.  Each
@@ -138,7 +138,7 @@ class MultiRun:
         return ok
 
 
-def run_config3(log=None, steps: int = 100, tune_steps: int = 20, space=None) -> dict:
+def run_default(log=None, steps: int = 100, tune_steps: int = 20, space=None) -> dict:
     work = make_workload()
     space = space or dict(pagesize=(4096, 16384, 65536), regionsize=(8, 16, 32), aggregation=(1, 16, 64))
     points = []
@@ -168,7 +168,7 @@ def run_config3(log=None, steps: int = 100, tune_steps: int = 20, space=None) ->
     ref_ms, ref_dc = ref.run(steps, ref_tun)
     ref.close()
     out = {
-        "workload": "config 3: 64 heaps x 256 MiB, log-uniform 1024..262144 slots (%d total), Zipf(1.2) sizes "
+        "workload": "multi-heap: 64 heaps x 256 MiB, log-uniform 1024..262144 slots (%d total), Zipf(1.2) sizes "
                     "16..65536 B, 100 graph-replayed steps of one persistent launch" % len(work.slot_heap),
         "mallocs_frees_per_s": float((dc[0] + dc[1]) / (ms / 1e3)), "ms_per_step": ms / steps,
         "mallocs": int(dc[0]), "frees": int(dc[1]), "nulls": int(dc[2]),
@@ -177,5 +177,5 @@ def run_config3(log=None, steps: int = 100, tune_steps: int = 20, space=None) ->
                                 "ms_per_step": ref_ms / steps, "nulls": int(ref_dc[2])},
     }
     if log:
-        log(f"config3: {out}")
+        log(f"multi-heap: {out}")
     return out

@@ -1,4 +1,4 @@
-"""Config 4: fragmentation / OOM stress.
+"""Fragmentation / OOM stress.
 
 Heap 64 GiB at pagesize 64 KiB.  2,097,152 lanes allocate bimodal sizes (80%
 uniform 16..256 B, 20% uniform 8..60 KiB) until the first null any of them
@@ -60,7 +60,7 @@ def _no_overlap(torch, offs, sizes):
     return int(((o[:-1] + s[:-1]) > o[1:]).sum().item()) == 0
 
 
-def run_config4(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles: int = 5, pagesize: int = 65536,
+def run_default(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles: int = 5, pagesize: int = 65536,
                 regionsize: int = 16, tuning: mmc.Tuning | None = None, log=None, cap: int | None = None,
                 first_null: bool = True) -> dict:
     """first_null: every lane stops at the first null any lane sees (the
@@ -120,7 +120,7 @@ def run_config4(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles
                    fill_s=t_fill, free_s=t_free, invariants_ok=bool(cyc_ok), slots_after_fill=slots)
         results.append(rec)
         if log:
-            log(f"config4 cycle {rec}")
+            log(f"oom stress cycle {rec}")
     # free everything that is left; the slot counts come back
     n = int(count.item())
     alloc.wave_free_batch(offs[:n].contiguous(), tuning=tuning)
@@ -128,7 +128,7 @@ def run_config4(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles
     slots_end = [alloc.getAvailableSlots(s) for s in probe]
     alloc.close()
     return {
-        "workload": f"config 4: heap {heap_bytes // GiB} GiB, pagesize {pagesize // 1024} KiB, {lanes} lanes fill "
+        "workload": f"OOM stress: heap {heap_bytes // GiB} GiB, pagesize {pagesize // 1024} KiB, {lanes} lanes fill "
                     f"until {'the first null' if first_null else 'each lane sees a null'} with 80% U[16,256] B / "
                     f"20% U[8,60] KiB, free a random half, {cycles} cycles",
         "cycles": results, "invariants_ok": bool(ok), "slots_restored": slots_end == slots0,

@@ -1,4 +1,4 @@
-"""Serving loop driver (config 5): Poisson arrivals, growing paged buffers,
+"""Serving loop driver: Poisson arrivals, growing paged buffers,
 admission by free heap bytes, p50/p99 step latency.
 
 The loop itself runs on the device (include/mallocmc_serve.h); this module
@@ -200,7 +200,7 @@ class ServeLoop:
 
 
 def run_default(log=None, duration_s: float = 30.0, check_sample: int = 64) -> dict:
-    """Config 5: 32 GiB heap, 2000 req/s Poisson over 30 s in 10 ms steps,
+    """The default serving workload: 32 GiB heap, 2000 req/s Poisson over 30 s in 10 ms steps,
     lognormal(6.2, 0.8) tokens (<= 8192) at 256 B/token, 4 KiB blocks of 16
     tokens, admission by free bytes.  Runs until every request completed."""
     import torch
@@ -232,7 +232,7 @@ def run_default(log=None, duration_s: float = 30.0, check_sample: int = 64) -> d
     busy = ms[:arrival_steps]  # steps while arrivals continue (steady state)
     total_s = float(ms.sum()) / 1e3
     out = {
-        "workload": f"config 5: Poisson 2000 req/s for {duration_s:.0f} s in 10 ms steps, lognormal(6.2,0.8) tokens "
+        "workload": f"serving loop: Poisson 2000 req/s for {duration_s:.0f} s in 10 ms steps, lognormal(6.2,0.8) tokens "
                     "<= 8192, 256 B/token, 4 KiB blocks, 32 GiB heap (one access block), pagesize 16 KiB",
         "timing": "device clock stamp at every step start; 100 steps per graph replay (one host launch "
                   "per 100 steps); p50/p99 over the arrival steps",

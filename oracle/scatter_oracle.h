@@ -15,7 +15,7 @@
  * What pins this restatement (tests/test_oracle.py): closed forms of the
  * geometry, the page capacities and the first serialized offset of every
  * size on a fresh heap; the slot counts and first offset a probe build of the
- * reference printed for config 1; brute-force streams on tiny heaps checking
+ * reference printed for 16,384 x malloc(16) on a 64 MiB heap; brute-force streams on tiny heaps checking
  * after every step that no allocations overlap, the wastefactor rule holds, a
  * page hands out its lowest free chunk, the count equals the live chunks and a
  * null means no page can take the request; the special cases of 0 bytes,

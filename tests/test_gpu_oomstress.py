@@ -1,4 +1,4 @@
-"""GPU test of the fragmentation / OOM stress cycle (config 4 shape on a
+"""GPU test of the fragmentation / OOM stress cycle (the 64 GiB workload's shape on a
 1 GiB heap): fill until null, free a random half, refill; invariants after
 every fill and the slot counts restored at the end.  The fill reached at the
 first null is compared with the reference protocol's (the oracle, serialized)
@@ -13,7 +13,7 @@ GiB = 1024 ** 3
 @pytest.mark.parametrize("split,first_null", [(0, False), (2, False), (2, True)])
 def test_oom_stress_cycles(mmc, cuda, split, first_null):
     from mallocmc_amd import oomstress
-    r = oomstress.run_config4(heap_bytes=1 * GiB, lanes=65536, cycles=3,
+    r = oomstress.run_default(heap_bytes=1 * GiB, lanes=65536, cycles=3,
                               tuning=mmc.Tuning(aggregation=64, class_split=split), log=print,
                               first_null=first_null)
     assert r["invariants_ok"], r
@@ -33,7 +33,7 @@ def test_oom_first_null_fill_close_to_reference_protocol(mmc, cuda, oracle_mod):
     from oom_fill import oracle_first_null_fill
     heap = 1024 ** 3
     ref = oracle_first_null_fill(oracle_mod, heap, cycles=3)
-    r = oomstress.run_config4(heap_bytes=heap, lanes=65536, cycles=3, log=print)
+    r = oomstress.run_default(heap_bytes=heap, lanes=65536, cycles=3, log=print)
     assert r["invariants_ok"] and r["slots_restored"], r
     for c, o in zip(r["cycles"], ref):
         print(f"fill gpu {c['fill_fraction']:.3f} reference protocol {o['fill_fraction']:.3f}", flush=True)

@@ -36,8 +36,8 @@ def _serial_pair(mmc, orc, cuda, cfg, heap_bytes, ops):
     return want, got, want_slots, got_slots
 
 
-def test_serial_config1_offsets(mmc, oracle_mod, cuda):
-    """Config 1: 16,384 x malloc(16) in thread order, then the frees."""
+def test_serial_16B_offsets(mmc, oracle_mod, cuda):
+    """16,384 x malloc(16) on a 64 MiB heap in thread order, then the frees."""
     cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=2 * MiB, regionsize=16, wastefactor=2,
                             resetfreedpages=True)
     n = 16384
@@ -120,8 +120,8 @@ def _check_no_overlap(torch, offsets, sizes, pad=16):
     return int(((off[:-1] + sz[:-1]) > off[1:]).sum().item())
 
 
-def test_concurrent_config1(mmc, cuda):
-    """Config 1 concurrently: 64 x 256 lanes malloc(16), write, verify, free;
+def test_concurrent_16B(mmc, cuda):
+    """64 x 256 lanes malloc(16) concurrently on a 64 MiB heap, write, verify, free;
     free-all restores the slot count (resetfreedpages on)."""
     torch = cuda
     cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=2 * MiB, regionsize=16, resetfreedpages=True)
@@ -142,7 +142,7 @@ def test_concurrent_config1(mmc, cuda):
 
 @pytest.mark.parametrize("dist", [0, 1])
 def test_concurrent_mixed_sizes_rounds(mmc, cuda, dist):
-    """Config 2 shape at 1/4 of the threads (262,144 lanes, 1 GiB heap, 16 KiB
+    """The mixed-size batch at 1/4 of the threads (262,144 lanes, 1 GiB heap, 16 KiB
     pages): three seeded rounds of malloc / write / verify / free; free-all
     restores the slot counts."""
     torch = cuda

@@ -37,10 +37,10 @@ TUNINGS = [
 
 
 @pytest.mark.parametrize("tk", range(len(TUNINGS)))
-def test_wave_config1(mmc, cuda, tk):
+def test_wave_16B(mmc, cuda, tk):
     torch = cuda
     t = mmc.Tuning(**TUNINGS[tk])
-    print(f"wave config1 tuning={t}", flush=True)
+    print(f"wave 16 B tuning={t}", flush=True)
     cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=2 * MiB, regionsize=16, resetfreedpages=True)
     a = mmc.ScatterAllocator(cfg, 64 * MiB)
     s0 = a.getAvailableSlots(16)

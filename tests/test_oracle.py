@@ -5,7 +5,8 @@ here (alpaka needs Boost, absent from the image), so the oracle is pinned by:
   * closed forms of the geometry (Scatter.hpp:962-1014), the page capacities
     (Scatter.hpp:365-383, 494-510) and the first serialized offset of each
     size on a fresh heap (Scatter.hpp:535-551 with smid = warpid = 0);
-  * the slot counts a probe build of the reference printed for config 1
+  * the slot counts a probe build of the reference printed for 16,384 x
+    malloc(16) on a 64 MiB heap at pagesize 4 KiB
     (4,031,488 -> 4,015,104 -> 4,031,488; first offset 458,752);
   * per-step invariants on brute-force random streams over tiny heaps, which
     hold whatever the bit tricks inside: no overlap, alignment, the
@@ -42,7 +43,7 @@ def capacity(P, c):
 
 # --------------------------------------------------------------- geometry
 GEOMETRY_TABLE = [
-    # (P, AB, R, heap) -> pages, regions, access blocks, pages per block  (closed forms)
+    # (P, AB, R, heap) -> pages, regions, access blocks, pages per block  (closed forms of Scatter.hpp:962-1014)
     ((4096, 2 * MiB, 16, 64 * MiB), (15872, 992, 31, 512)),
     ((16384, 2 * GiB, 16, 4 * GiB), (261936, 16371, 1, 261936)),
     ((16384, 2 * MiB, 8, 256 * MiB), (16256, 2032, 127, 128)),
@@ -115,7 +116,7 @@ def test_empty_heap_slots_closed_form(oracle_mod, P):
             assert h.available_slots(k * P) == n // k
 
 
-def test_config1_probe_values(oracle_mod):
+def test_probe_values_16B_64MiB(oracle_mod):
     """Values printed by a probe build of the reference on its CPU backend."""
     cfg = oracle_mod.make_config(pagesize=4096, accessblocksize=2 * MiB, resetfreedpages=True)
     with oracle_mod.OracleHeap(cfg, 64 * MiB) as h:
