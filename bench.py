@@ -12,7 +12,9 @@ Forms:
                 request, one launch per phase, pagesize 16 KiB, regionsize 16;
   * batched  -- wave-cooperative claims with the geometry and wave tuning the
                 autotuner picks, malloc+free of a step captured in one hipGraph
-                and replayed.  This is `value`.
+                and replayed.  This is `value`.  The same form on the plain
+                form's geometry (default tuning, eager launches) is reported
+                beside it as batched_reference_geometry.
 
 Also reported: the device-atomic ceiling (microbenchmark), the roofline of
 the dominant kernel, the CPU baseline (the oracle on this host's cores, the
@@ -99,6 +101,37 @@ def measure_plain(torch, mmc, batches, steps):
     return dict(value=2 * N_LANES * steps / (total_ms / 1e3), ms_per_step=total_ms / steps,
                 malloc_ms=mall_ms / steps, nulls=nulls, steps=steps,
                 config={"pagesize": 16384, "regionsize": 16, "path": "per-lane", "launch": "plain"})
+
+
+def measure_batched_reference_geometry(torch, mmc, batches, steps):
+    """The batched form on the plain form's configuration (pagesize 16 KiB,
+    regionsize 16, the reference's values for this workload, no pagesize-leak
+    fix), default tuning: what the wave path gives without the autotuner."""
+    cfg = mmc.ScatterConfig(pagesize=16384, accessblocksize=2 * GiB, regionsize=16, resetfreedpages=True)
+    a = mmc.ScatterAllocator(cfg, HEAP)
+    tun = mmc.Tuning()
+    offs = torch.empty(N_LANES, dtype=torch.int64, device="cuda")
+    for b in batches:
+        a.wave_malloc_batch(b, offs, tuning=tun)
+        a.wave_free_batch(offs, tuning=tun)
+    torch.cuda.synchronize()
+    total_ms, nulls = 0.0, 0
+    for i in range(steps):
+        sizes = batches[i % len(batches)]
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        a.wave_malloc_batch(sizes, offs, tuning=tun)
+        a.wave_free_batch(offs, tuning=tun)
+        e1.record()
+        torch.cuda.synchronize()
+        total_ms += e0.elapsed_time(e1)
+        if i < len(batches):
+            nulls += int((offs < 0).sum().item())
+    a.close()
+    return dict(value=2 * N_LANES * steps / (total_ms / 1e3), ms_per_step=total_ms / steps, steps=steps,
+                nulls_first_steps=nulls,
+                config={"pagesize": 16384, "regionsize": 16, "aggregation": 64, "xcd_slices": 0,
+                        "path": "wave-cooperative", "launch": "eager, one malloc + one free launch per step"})
 
 
 def cpu_baseline(sizes_np):
@@ -260,6 +293,9 @@ def main():
     torch.cuda.synchronize()
     del src, dst
 
+    ref_geom = measure_batched_reference_geometry(torch, mmc, batches, args.steps)
+    log(f"batched at the reference geometry: {ref_geom}")
+
     plain = None
     if not args.no_plain:
         plain = measure_plain(torch, mmc, batches, args.plain_steps)
@@ -313,6 +349,7 @@ def main():
         "nulls_per_step": nulls,
         "invariants_ok": bool(invariants_ok),
         "slots_restored": bool(slots_restored),
+        "batched_reference_geometry": ref_geom,
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": {
