@@ -192,7 +192,7 @@ def main():
     base = mmc.ScatterConfig(accessblocksize=2 * GiB, resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
     if args.no_tune:
         # the point the autotuner picks on this workload (pagesize 64 KiB, regionsize 32)
-        best = autotune.TunePoint(pagesize=65536, regionsize=32, aggregation=64, persistent=False, xcd_slices=8)
+        best = autotune.TunePoint(pagesize=65536, regionsize=32, aggregation=64, persistent=False, xcd_slices=0)
         tuned_points = 0
     else:
         t0 = time.time()

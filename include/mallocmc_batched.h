@@ -31,6 +31,8 @@ typedef struct mmc_tuning {
     uint32_t no_partition; /* 1: lane i of a non-persistent launch serves request i (0, the default:
                               each block of 256 first reorders its requests by size, its frees by
                               page, so that a wave meets few size classes / pages) */
+    uint32_t block_threads; /* threads per block of the non-persistent wave malloc and free, the
+                               requests one partition sorts: 256, 512 or 1024 (0 = 1024) */
 } mmc_tuning;
 
 int mmc_default_tuning(mmc_tuning* t);

@@ -58,6 +58,7 @@ class MmcTuning(ctypes.Structure):
         ("flat_single", ctypes.c_uint32),
         ("no_skip", ctypes.c_uint32),
         ("no_partition", ctypes.c_uint32),
+        ("block_threads", ctypes.c_uint32),
     ]
 
 
@@ -68,7 +69,8 @@ class Tuning:
     whether flat pages are claimed one at a time (flat_single), whether
     the region scan walks every window (no_skip) and whether a block serves
     its requests in arrival order instead of partitioned by size
-    (no_partition)."""
+    (no_partition), and the threads per block of the non-persistent wave
+    malloc and free (block_threads: 256, 512 or 1024; 0 = 1024)."""
     aggregation: int = 64
     xcd_slices: int = 0
     persistent: bool = False
@@ -77,11 +79,12 @@ class Tuning:
     flat_single: bool = False
     no_skip: bool = False
     no_partition: bool = False
+    block_threads: int = 0
 
     def to_c(self) -> MmcTuning:
         return MmcTuning(self.aggregation, self.xcd_slices, int(bool(self.persistent)), self.waves_per_cu,
                          self.class_split, int(bool(self.flat_single)), int(bool(self.no_skip)),
-                         int(bool(self.no_partition)))
+                         int(bool(self.no_partition)), self.block_threads)
 
 
 class MmcStepDesc(ctypes.Structure):

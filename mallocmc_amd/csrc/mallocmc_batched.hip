@@ -28,6 +28,14 @@ namespace
         return t && t->no_partition ? 0u : 1u;
     }
 
+    //! Threads per block of the one-request-per-lane kernels: 256, 512 or
+    //! 1024 (0 = 1024).
+    u32 blockOf(const mmc_tuning* t)
+    {
+        const u32 b = t ? t->block_threads : 0u;
+        return b == 0u || b >= 1024u ? 1024u : (b >= 512u ? 512u : 256u);
+    }
+
     WaveTuning toTuning(const mmc_tuning* t)
     {
         WaveTuning w;
@@ -65,38 +73,48 @@ namespace
         return 223u - (8u * (e - 2u) + ((x >> (e - 3u)) & 7u));  // 0..215
     }
 
-    //! Reorders the 256 values of a block by an 8-bit key (LDS counting
-    //! sort, unstable).  Returns the value this thread receives and the
-    //! block slot it came from.
-    template<class T>
+    //! Reorders the B values of a block (B = 256, 512 or 1024 threads) by an
+    //! 8-bit key (LDS counting sort, unstable).  Returns the value this
+    //! thread receives and the block slot it came from.
+    template<class T, u32 B>
     __device__ void partitionByKey(u32 key, T value, T& outValue, u32& outSlot)
     {
+        static_assert(B % 256u == 0u && B <= 1024u, "a block of 256, 512 or 1024 threads");
         __shared__ u32 count[kSizeBuckets];
         __shared__ u32 start[kSizeBuckets];
         __shared__ u32 waveTotal[4];
-        __shared__ T sortedValue[256];
-        __shared__ u32 sortedSlot[256];
+        __shared__ T sortedValue[B];
+        __shared__ u32 sortedSlot[B];
         const u32 tid = threadIdx.x;
-        count[tid] = 0u;
+        if(tid < kSizeBuckets)
+            count[tid] = 0u;
         __syncthreads();
         const u32 rank = atomicAdd(&count[key], 1u);
         __syncthreads();
-        // exclusive scan of the 256 bucket counts: per wave, then across waves
-        const u32 c = count[tid];
-        u32 incl = c;
-        for(u32 d = 1u; d < 64u; d <<= 1)
+        // exclusive scan of the 256 bucket counts by the first four waves:
+        // per wave, then across them
+        u32 c = 0u, incl = 0u;
+        if(tid < kSizeBuckets)
         {
-            const u32 up = __shfl_up(incl, d, 64);
-            if(hw::laneid() >= d)
-                incl += up;
+            c = count[tid];
+            incl = c;
+            for(u32 d = 1u; d < 64u; d <<= 1)
+            {
+                const u32 up = __shfl_up(incl, d, 64);
+                if(hw::laneid() >= d)
+                    incl += up;
+            }
+            if(hw::laneid() == 63u)
+                waveTotal[tid / 64u] = incl;
         }
-        if(hw::laneid() == 63u)
-            waveTotal[tid / 64u] = incl;
         __syncthreads();
-        u32 before = 0u;
-        for(u32 w = 0; w < tid / 64u; ++w)
-            before += waveTotal[w];
-        start[tid] = before + incl - c;
+        if(tid < kSizeBuckets)
+        {
+            u32 before = 0u;
+            for(u32 w = 0; w < tid / 64u; ++w)
+                before += waveTotal[w];
+            start[tid] = before + incl - c;
+        }
         __syncthreads();
         const u32 pos = start[key] + rank;
         sortedValue[pos] = value;
@@ -106,22 +124,23 @@ namespace
         outSlot = sortedSlot[tid];
     }
 
-    //! The 256 requests of a block are served by its four waves; the whole
+    //! The B requests of a block are served by its B/64 waves; the whole
     //! wave cooperates.  With `partition` the block's requests are first
     //! reordered by size bucket (partitionByKey), and each lane serves the
     //! request its slot received: uniformly mixed sizes put about six size
     //! classes in a wave of 64 arrivals, and every class costs the wave one
-    //! scan and one claim.  Without it lane i serves request i.
-    template<class Dist>
-    __global__ __launch_bounds__(256) void waveMallocKernel(ScatterHeap h, WaveTuning t, const u32* sizes,
-                                                            u64* offsets, u64 n, u32 partition)
+    //! scan and one claim; the larger the block, the fewer waves straddle
+    //! two classes.  Without it lane i serves request i.
+    template<class Dist, u32 B>
+    __global__ __launch_bounds__(B) void waveMallocKernel(ScatterHeap h, WaveTuning t, const u32* sizes,
+                                                          u64* offsets, u64 n, u32 partition)
     {
-        const u64 base = static_cast<u64>(blockIdx.x) * blockDim.x;
+        const u64 base = static_cast<u64>(blockIdx.x) * B;
         const u64 i = base + threadIdx.x;
         u32 bytes = i < n ? sizes[i] : 0u;
         u32 slot = threadIdx.x;
         if(partition != 0u)
-            partitionByKey(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
+            partitionByKey<u32, B>(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
         const WaveAlloc wa{h, t};
         void* p = wa.malloc<Dist>(bytes);
         if(base + slot < n)
@@ -131,16 +150,17 @@ namespace
     //! With `partition` the block's frees are first reordered by page index
     //! mod 256, so that the frees of one page gather in one wave and share
     //! its atomics.
-    __global__ __launch_bounds__(256) void waveFreeKernel(ScatterHeap h, WaveTuning t, const u64* offsets, u64 n,
-                                                          u32 partition)
+    template<u32 B>
+    __global__ __launch_bounds__(B) void waveFreeKernel(ScatterHeap h, WaveTuning t, const u64* offsets, u64 n,
+                                                        u32 partition)
     {
-        const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+        const u64 i = static_cast<u64>(blockIdx.x) * B + threadIdx.x;
         u64 off = i < n ? offsets[i] : kNullOffset;
         if(partition != 0u)
         {
             const u32 key = off == kNullOffset ? 255u : (h.pageOf(off) & 255u);
             u32 slot = 0u;
-            partitionByKey(key, off, off, slot);
+            partitionByKey<u64, B>(key, off, off, slot);
         }
         const WaveAlloc wa{h, t};
         wa.free(off == kNullOffset ? nullptr : h.pages + off);
@@ -420,6 +440,7 @@ int mmc_default_tuning(mmc_tuning* t)
     t->flat_single = 0u;
     t->no_skip = 0u;
     t->no_partition = 0u;
+    t->block_threads = 0u;
     return MMC_OK;
 }
 
@@ -448,13 +469,33 @@ int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_o
     }
     else
     {
-        const unsigned blocks = mmc::gridFor(n, 256);
+        const u32 part = partitionOf(tuning);
+        const u32 bs = blockOf(tuning);
+        const unsigned blocks = mmc::gridFor(n, bs);
         if(xm)
-            hipLaunchKernelGGL(waveMallocKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, heap->view, t, d_sizes,
-                               d_offsets, n, partitionOf(tuning));
+        {
+            if(bs == 1024u)
+                hipLaunchKernelGGL((waveMallocKernel<XMallocSIMD64, 1024>), dim3(blocks), dim3(1024), 0, s,
+                                   heap->view, t, d_sizes, d_offsets, n, part);
+            else if(bs == 512u)
+                hipLaunchKernelGGL((waveMallocKernel<XMallocSIMD64, 512>), dim3(blocks), dim3(512), 0, s, heap->view,
+                                   t, d_sizes, d_offsets, n, part);
+            else
+                hipLaunchKernelGGL((waveMallocKernel<XMallocSIMD64, 256>), dim3(blocks), dim3(256), 0, s, heap->view,
+                                   t, d_sizes, d_offsets, n, part);
+        }
         else
-            hipLaunchKernelGGL(waveMallocKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, heap->view, t,
-                               d_sizes, d_offsets, n, partitionOf(tuning));
+        {
+            if(bs == 1024u)
+                hipLaunchKernelGGL((waveMallocKernel<NoopDistribution, 1024>), dim3(blocks), dim3(1024), 0, s,
+                                   heap->view, t, d_sizes, d_offsets, n, part);
+            else if(bs == 512u)
+                hipLaunchKernelGGL((waveMallocKernel<NoopDistribution, 512>), dim3(blocks), dim3(512), 0, s,
+                                   heap->view, t, d_sizes, d_offsets, n, part);
+            else
+                hipLaunchKernelGGL((waveMallocKernel<NoopDistribution, 256>), dim3(blocks), dim3(256), 0, s,
+                                   heap->view, t, d_sizes, d_offsets, n, part);
+        }
     }
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "wave malloc launch");
@@ -473,8 +514,18 @@ int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, c
         hipLaunchKernelGGL(persistentFreeKernel, dim3(persistentBlocks(tuning)), dim3(256), 0, s, heap->view, t,
                            d_offsets, n, mmc::heapCursor(heap));
     else
-        hipLaunchKernelGGL(waveFreeKernel, dim3(mmc::gridFor(n, 256)), dim3(256), 0, s, heap->view, t, d_offsets, n,
-                           partitionOf(tuning));
+    {
+        const u32 bs = blockOf(tuning);
+        if(bs == 1024u)
+            hipLaunchKernelGGL(waveFreeKernel<1024>, dim3(mmc::gridFor(n, 1024)), dim3(1024), 0, s, heap->view, t,
+                               d_offsets, n, partitionOf(tuning));
+        else if(bs == 512u)
+            hipLaunchKernelGGL(waveFreeKernel<512>, dim3(mmc::gridFor(n, 512)), dim3(512), 0, s, heap->view, t,
+                               d_offsets, n, partitionOf(tuning));
+        else
+            hipLaunchKernelGGL(waveFreeKernel<256>, dim3(mmc::gridFor(n, 256)), dim3(256), 0, s, heap->view, t,
+                               d_offsets, n, partitionOf(tuning));
+    }
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "wave free launch");
 }
