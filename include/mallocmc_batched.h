@@ -29,8 +29,8 @@ typedef struct mmc_tuning {
     uint32_t no_skip;      /* 1: walk every window of the region scan (0, the default: after a window
                               without candidate pages, skip ahead 256 pages per round trip) */
     uint32_t no_partition; /* 1: lane i of a non-persistent launch serves request i (0, the default:
-                              each block of 256 first reorders its requests by size, its frees by
-                              page, so that a wave meets few size classes / pages) */
+                              each block first reorders its requests by size, its frees by page,
+                              so that a wave meets few size classes / pages) */
     uint32_t block_threads; /* threads per block of the non-persistent wave malloc and free, the
                                requests one partition sorts: 256, 512 or 1024 (0 = 1024) */
 } mmc_tuning;
@@ -106,9 +106,15 @@ typedef struct mmc_step_desc {
     uint32_t* d_sizes;    /* size of the live allocation per slot (for verification) */
     uint64_t* d_step;
     uint64_t* d_counters;
-    uint32_t* d_tile_cursor; /* work-queue cursor, reset by the launch itself */
+    uint32_t* d_tile_cursor; /* uint32[2], zero before the first launch; each launch leaves it at zero */
 } mmc_step_desc;
 
+/* One step.  tuning NULL or tuning->persistent = 1: persistent waves pull the
+ * tiles from d_tile_cursor[0].  tuning->persistent = 0: one workgroup of four
+ * waves per four consecutive tiles; when the four lie in one heap the
+ * workgroup first reorders its frees by page and its mallocs by size (unless
+ * tuning->no_partition).  Which form is faster depends on the heaps' fill
+ * (the multi-heap driver autotunes it). */
 int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tuning, void* stream);
 
 #ifdef __cplusplus

@@ -406,8 +406,10 @@ class ScatterAllocator:
 
 
 class MultiHeap:
-    """Many independent heaps served by one persistent launch per step
-    (mmc_multi_*).  The heaps stay owned by their ScatterAllocator objects."""
+    """Many independent heaps served by one launch per step (mmc_multi_*):
+    persistent waves pulling 64-slot tiles (Tuning(persistent=True), the
+    default here) or one workgroup per four tiles (persistent=False).  The
+    heaps stay owned by their ScatterAllocator objects."""
 
     def __init__(self, allocators):
         self.allocators = list(allocators)

@@ -1,5 +1,6 @@
 // Batched entry points: wave-cooperative malloc/free over one heap (plain or
-// persistent launch) and the persistent multi-heap step kernel.
+// persistent launch) and the multi-heap step kernels (persistent, or one
+// workgroup per four tiles).
 // Declarations: include/mallocmc_batched.h.
 #include "mallocmc_batched.h"
 
@@ -333,6 +334,114 @@ namespace
             atom::add(d.d_step, u64{1});
     }
 
+    //! One step over many heaps, one workgroup per four consecutive 64-slot
+    //! tiles (no work queue).  The draws of a slot are those of
+    //! multiStepKernel.  When the four tiles lie in one heap the workgroup
+    //! first reorders its frees by page and its mallocs by size
+    //! (partitionByKey), as the one-heap batch kernels do; the barriers all
+    //! come before the wave calls, so a slow wave holds up no other.  The
+    //! last workgroup to finish resets the finish counter and advances the
+    //! step.
+    template<class Dist>
+    __global__ __launch_bounds__(256) void multiStepBlockKernel(const ScatterHeap* heaps, mmc_step_desc d,
+                                                                WaveTuning t, bool partition)
+    {
+        __shared__ u32 sHeap[4];
+        __shared__ u32 sFirst[4];
+        __shared__ u32 sLast;
+        const u64 step = *d.d_step;
+        const u32 me = hw::laneid();
+        const u32 w = threadIdx.x / 64u;
+        const u32 tile = 4u * blockIdx.x + w;
+        const bool haveTile = tile < d.n_tiles;
+        const u32 first = haveTile ? __builtin_amdgcn_readfirstlane(d.d_tiles[tile]) : 0u;
+        const u32 heapIdx = haveTile ? __builtin_amdgcn_readfirstlane(d.d_slot_heap[first]) : 0xffffffffu;
+        if(me == 0u)
+        {
+            sHeap[w] = heapIdx;
+            sFirst[w] = first;
+        }
+        __syncthreads();
+        const bool blockWide = partition && sHeap[0] != 0xffffffffu && sHeap[1] == sHeap[0]
+            && sHeap[2] == sHeap[0] && sHeap[3] == sHeap[0];
+        const u32 s = first + me;
+        const bool valid = haveTile && s < d.n_slots && d.d_slot_heap[s] == heapIdx;
+        // the view is the same for the whole wave: keep it in scalar registers
+        const ScatterHeap h = uniformCopy(heaps[__builtin_amdgcn_readfirstlane(haveTile ? heapIdx : sHeap[0])]);
+        const WaveAlloc wa{h, t};
+        const u64 r = mix64(d.seed ^ mix64(step * 0x100000001B3ull + s));
+        const u32 coin = static_cast<u32>(r % 1000u);
+        const u64 off = valid ? d.d_offsets[s] : kNullOffset;
+        const bool live = off != kNullOffset;
+        const bool doFree = valid && live && coin < d.p_free_permille;
+        const bool doAlloc = valid && !live && coin < d.p_alloc_permille;
+        u32 bytes = 0u;
+        if(doAlloc)
+        {
+            const u32 u = static_cast<u32>(r >> 32);
+            u32 k = 0u;
+            while(k + 1u < d.n_sizes && u > d.d_size_cdf[k])
+                ++k;
+            bytes = d.d_size_table[k];
+        }
+        u64 nMalloc = 0, nFree = 0, nNull = 0;
+        if(doFree)
+        {
+            d.d_offsets[s] = kNullOffset;
+            d.d_sizes[s] = 0u;
+            ++nFree;
+        }
+        u64 freeOff = doFree ? off : kNullOffset;
+        u32 slot = threadIdx.x;
+        if(blockWide)
+        {
+            const u32 key = freeOff == kNullOffset ? 255u : (h.pageOf(freeOff) & 255u);
+            partitionByKey<u64, 256>(key, freeOff, freeOff, slot);
+            slot = threadIdx.x;
+            partitionByKey<u32, 256>(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
+        }
+        wa.free(freeOff == kNullOffset ? nullptr : h.pages + freeOff);
+        void* p = wa.malloc<Dist>(bytes);
+        if(bytes != 0u)
+        {
+            // the slot whose request this lane served
+            const u32 target = blockWide ? sFirst[slot / 64u] + (slot % 64u) : s;
+            ++nMalloc;
+            if(p == nullptr)
+                ++nNull;
+            d.d_offsets[target] = h.offsetOf(p);
+            d.d_sizes[target] = p == nullptr ? 0u : bytes;
+        }
+        for(int x = 32; x > 0; x >>= 1)
+        {
+            nMalloc += __shfl_xor(nMalloc, x, 64);
+            nFree += __shfl_xor(nFree, x, 64);
+            nNull += __shfl_xor(nNull, x, 64);
+        }
+        if(me == 0u)
+        {
+            if(nMalloc)
+                atom::add(&d.d_counters[0], nMalloc);
+            if(nFree)
+                atom::add(&d.d_counters[1], nFree);
+            if(nNull)
+                atom::add(&d.d_counters[2], nNull);
+            if(haveTile)
+                atom::add(&d.d_counters[3], u64{1});
+        }
+        // the last workgroup to finish advances the step (every workgroup read
+        // it at its start) and resets the finish counter for the next launch
+        __syncthreads();
+        if(threadIdx.x == 0u)
+            sLast = atom::add(&d.d_tile_cursor[1], 1u) == gridDim.x - 1u ? 1u : 0u;
+        __syncthreads();
+        if(sLast && threadIdx.x == 0u)
+        {
+            atom::store(&d.d_tile_cursor[1], 0u);
+            atom::add(d.d_step, u64{1});
+        }
+    }
+
     //! CU count of the current device, queried once per device (never inside
     //! a stream capture after the first call).
     int cuCount()
@@ -595,11 +704,27 @@ int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tu
         return MMC_OK;
     const WaveTuning t = toTuning(tuning);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const unsigned blocks = persistentBlocks(tuning);
-    if(m->distribution == MMC_DIST_XMALLOCSIMD)
-        hipLaunchKernelGGL(multiStepKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc, t);
+    const bool xm = m->distribution == MMC_DIST_XMALLOCSIMD;
+    if(!tuning || tuning->persistent)
+    {
+        const unsigned blocks = persistentBlocks(tuning);
+        if(xm)
+            hipLaunchKernelGGL(multiStepKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc, t);
+        else
+            hipLaunchKernelGGL(multiStepKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc,
+                               t);
+    }
     else
-        hipLaunchKernelGGL(multiStepKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc, t);
+    {
+        const unsigned blocks = (desc->n_tiles + 3u) / 4u;
+        const bool part = partitionOf(tuning);
+        if(xm)
+            hipLaunchKernelGGL(multiStepBlockKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc,
+                               t, part);
+        else
+            hipLaunchKernelGGL(multiStepBlockKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, m->dHeaps,
+                               *desc, t, part);
+    }
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "multi step launch");
 }

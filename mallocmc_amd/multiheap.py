@@ -1,4 +1,4 @@
-"""Many independent heaps served by one persistent launch per step.
+"""Many independent heaps served by one launch per step.
 
 64 instances; instance i gets floor(1024 * 256**u_i) request slots (u_i
 uniform, so log-uniform in [1024, 262144]) and its own 256 MiB heap.  Each
@@ -6,7 +6,8 @@ step every live slot frees with probability 1/2 and every empty slot
 allocates with probability 1/2 a size drawn from Zipf(1.2) over the powers of
 two 16..65536 B (the steady state keeps about half the slots live).  100
 steps, each one graph replay of mmc_multi_step.  The autotuner sweeps
-pagesize {4K, 16K, 64K} x regionsize {8, 16, 32} x aggregation {1, 16, 64}.
+pagesize {4K, 16K, 64K} x regionsize {8, 16, 32} x aggregation {1, 16, 64} x
+launch form {persistent waves pulling tiles, one workgroup per four tiles}.
 """
 from __future__ import annotations
 
@@ -140,25 +141,26 @@ class MultiRun:
 
 def run_default(log=None, steps: int = 100, tune_steps: int = 20, space=None) -> dict:
     work = make_workload()
-    space = space or dict(pagesize=(4096, 16384, 65536), regionsize=(8, 16, 32), aggregation=(1, 16, 64))
+    space = space or dict(pagesize=(4096, 16384, 65536), regionsize=(8, 16, 32), aggregation=(1, 16, 64),
+                          persistent=(True, False))
     points = []
     t0 = time.time()
     for P, R in itertools.product(space["pagesize"], space["regionsize"]):
-        for agg in space["aggregation"]:
+        for agg, persistent in itertools.product(space["aggregation"], space["persistent"]):
             run = MultiRun(work, P, R)
-            tun = mmc.Tuning(aggregation=agg, persistent=True, xcd_slices=8)
+            tun = mmc.Tuning(aggregation=agg, persistent=persistent, xcd_slices=8)
             run.run(3, tun)  # warm up into the steady state
             ms, dc = run.run(tune_steps, tun)
             rate = (dc[0] + dc[1]) / (ms / 1e3)
-            points.append(dict(pagesize=P, regionsize=R, aggregation=agg, ops_per_s=float(rate),
-                               null_frac=float(dc[2] / max(1, dc[0]))))
+            points.append(dict(pagesize=P, regionsize=R, aggregation=agg, persistent=persistent,
+                               ops_per_s=float(rate), null_frac=float(dc[2] / max(1, dc[0]))))
             run.close()
             if log:
                 log(f"multi tune {points[-1]}")
     best = max(points, key=lambda p: p["ops_per_s"])
     tune_s = time.time() - t0
     run = MultiRun(work, best["pagesize"], best["regionsize"])
-    tun = mmc.Tuning(aggregation=best["aggregation"], persistent=True, xcd_slices=8)
+    tun = mmc.Tuning(aggregation=best["aggregation"], persistent=best["persistent"], xcd_slices=8)
     ms, dc = run.run(steps, tun)
     ok = run.check_invariants()
     run.close()
@@ -169,7 +171,7 @@ def run_default(log=None, steps: int = 100, tune_steps: int = 20, space=None) ->
     ref.close()
     out = {
         "workload": "multi-heap: 64 heaps x 256 MiB, log-uniform 1024..262144 slots (%d total), Zipf(1.2) sizes "
-                    "16..65536 B, 100 graph-replayed steps of one persistent launch" % len(work.slot_heap),
+                    "16..65536 B, 100 graph-replayed steps of one launch" % len(work.slot_heap),
         "mallocs_frees_per_s": float((dc[0] + dc[1]) / (ms / 1e3)), "ms_per_step": ms / steps,
         "mallocs": int(dc[0]), "frees": int(dc[1]), "nulls": int(dc[2]),
         "tuned": best, "autotune_points": len(points), "autotune_s": tune_s, "invariants_ok": bool(ok),

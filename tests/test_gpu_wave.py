@@ -274,15 +274,19 @@ def _check_live(torch, allocs, dev, slot_heap):
         assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0, r
 
 
-def test_multi_heap_steps_and_graph_replay(mmc, cuda):
+@pytest.mark.parametrize("persistent", [True, False])
+def test_multi_heap_steps_and_graph_replay(mmc, cuda, persistent):
     """8 heaps, 20 steps eager, then 20 steps replayed from a captured graph;
     the live set follows the seeded coins exactly and free-all restores every
-    heap's slot count."""
+    heap's slot count.  Both launch forms: persistent waves pulling tiles, and
+    one workgroup per four tiles (which reorders a one-heap workgroup's
+    requests by page and size)."""
     torch = cuda
     nheaps, per = 8, 4096
     allocs, multi, dev, d, slot_heap = _step_setup(torch, mmc, nheaps, per, 64 * MiB, 16384)
     s0 = [a.getAvailableSlots(64) for a in allocs]
-    tun = mmc.Tuning(persistent=True, aggregation=64)
+    tun = mmc.Tuning(persistent=persistent, aggregation=64)
+    print(f"multi-heap steps persistent={persistent}", flush=True)
     live_hist = []
     for _ in range(20):
         multi.step(d, tun)
@@ -334,9 +338,9 @@ def test_multi_heap_steps_with_multipage_requests(mmc, cuda):
     nheaps, per = 4, 8192
     allocs, multi, dev, d, slot_heap = _step_setup(torch, mmc, nheaps, per, 128 * MiB, 4096, seed=11, nsizes=13)
     s0 = [[a.getAvailableSlots(s) for s in (64, 4096, 8 * 4096)] for a in allocs]
-    for agg in (1, 64):
-        tun = mmc.Tuning(persistent=True, aggregation=agg)
-        print(f"multipage steps aggregation={agg}", flush=True)
+    for agg, persistent in ((1, True), (64, True), (64, False)):
+        tun = mmc.Tuning(persistent=persistent, aggregation=agg)
+        print(f"multipage steps aggregation={agg} persistent={persistent}", flush=True)
         for _ in range(20):
             multi.step(d, tun)
         torch.cuda.synchronize()
