@@ -31,8 +31,9 @@ typedef struct mmc_tuning {
     uint32_t no_partition; /* 1: lane i of a non-persistent launch serves request i (0, the default:
                               each block first reorders its requests by size, its frees by page,
                               so that a wave meets few size classes / pages) */
-    uint32_t block_threads; /* threads per block of the non-persistent wave malloc and free, the
-                               requests one partition sorts: 256, 512 or 1024 (0 = 1024) */
+    uint32_t block_threads; /* threads per block of the non-persistent wave malloc and free and of
+                               the non-persistent multi-heap step, the requests one partition
+                               sorts: 256, 512 or 1024 (0 = 1024) */
 } mmc_tuning;
 
 int mmc_default_tuning(mmc_tuning* t);
@@ -110,10 +111,10 @@ typedef struct mmc_step_desc {
 } mmc_step_desc;
 
 /* One step.  tuning NULL or tuning->persistent = 1: persistent waves pull the
- * tiles from d_tile_cursor[0].  tuning->persistent = 0: one workgroup of four
- * waves per four consecutive tiles; when the four lie in one heap the
- * workgroup first reorders its frees by page and its mallocs by size (unless
- * tuning->no_partition).  Which form is faster depends on the heaps' fill
+ * tiles from d_tile_cursor[0].  tuning->persistent = 0: one workgroup of
+ * tuning->block_threads threads per block_threads/64 consecutive tiles; when
+ * they lie in one heap the workgroup first reorders its frees by page and its
+ * mallocs by size (unless tuning->no_partition).  Which form is faster depends on the heaps' fill
  * (the multi-heap driver autotunes it). */
 int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tuning, void* stream);
 

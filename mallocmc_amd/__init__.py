@@ -70,7 +70,8 @@ class Tuning:
     the region scan walks every window (no_skip) and whether a block serves
     its requests in arrival order instead of partitioned by size
     (no_partition), and the threads per block of the non-persistent wave
-    malloc and free (block_threads: 256, 512 or 1024; 0 = 1024)."""
+    malloc and free and of the non-persistent multi-heap step
+    (block_threads: 256, 512 or 1024; 0 = 1024)."""
     aggregation: int = 64
     xcd_slices: int = 0
     persistent: bool = False
@@ -408,7 +409,8 @@ class ScatterAllocator:
 class MultiHeap:
     """Many independent heaps served by one launch per step (mmc_multi_*):
     persistent waves pulling 64-slot tiles (Tuning(persistent=True), the
-    default here) or one workgroup per four tiles (persistent=False).  The
+    default here) or one workgroup per block_threads/64 tiles
+    (persistent=False).  The
     heaps stay owned by their ScatterAllocator objects."""
 
     def __init__(self, allocators):

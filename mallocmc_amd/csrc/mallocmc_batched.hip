@@ -1,6 +1,6 @@
 // Batched entry points: wave-cooperative malloc/free over one heap (plain or
 // persistent launch) and the multi-heap step kernels (persistent, or one
-// workgroup per four tiles).
+// workgroup per 4, 8 or 16 tiles).
 // Declarations: include/mallocmc_batched.h.
 #include "mallocmc_batched.h"
 
@@ -334,25 +334,26 @@ namespace
             atom::add(d.d_step, u64{1});
     }
 
-    //! One step over many heaps, one workgroup per four consecutive 64-slot
+    //! One step over many heaps, one workgroup per B/64 consecutive 64-slot
     //! tiles (no work queue).  The draws of a slot are those of
-    //! multiStepKernel.  When the four tiles lie in one heap the workgroup
+    //! multiStepKernel.  When all its tiles lie in one heap the workgroup
     //! first reorders its frees by page and its mallocs by size
     //! (partitionByKey), as the one-heap batch kernels do; the barriers all
     //! come before the wave calls, so a slow wave holds up no other.  The
     //! last workgroup to finish resets the finish counter and advances the
     //! step.
-    template<class Dist>
-    __global__ __launch_bounds__(256) void multiStepBlockKernel(const ScatterHeap* heaps, mmc_step_desc d,
-                                                                WaveTuning t, bool partition)
+    template<class Dist, u32 B>
+    __global__ __launch_bounds__(B) void multiStepBlockKernel(const ScatterHeap* heaps, mmc_step_desc d,
+                                                              WaveTuning t, bool partition)
     {
-        __shared__ u32 sHeap[4];
-        __shared__ u32 sFirst[4];
+        constexpr u32 W = B / 64u;
+        __shared__ u32 sHeap[W];
+        __shared__ u32 sFirst[W];
         __shared__ u32 sLast;
         const u64 step = *d.d_step;
         const u32 me = hw::laneid();
         const u32 w = threadIdx.x / 64u;
-        const u32 tile = 4u * blockIdx.x + w;
+        const u32 tile = W * blockIdx.x + w;
         const bool haveTile = tile < d.n_tiles;
         const u32 first = haveTile ? __builtin_amdgcn_readfirstlane(d.d_tiles[tile]) : 0u;
         const u32 heapIdx = haveTile ? __builtin_amdgcn_readfirstlane(d.d_slot_heap[first]) : 0xffffffffu;
@@ -362,8 +363,9 @@ namespace
             sFirst[w] = first;
         }
         __syncthreads();
-        const bool blockWide = partition && sHeap[0] != 0xffffffffu && sHeap[1] == sHeap[0]
-            && sHeap[2] == sHeap[0] && sHeap[3] == sHeap[0];
+        bool blockWide = partition && sHeap[0] != 0xffffffffu;
+        for(u32 i = 1; i < W; ++i)
+            blockWide = blockWide && sHeap[i] == sHeap[0];
         const u32 s = first + me;
         const bool valid = haveTile && s < d.n_slots && d.d_slot_heap[s] == heapIdx;
         // the view is the same for the whole wave: keep it in scalar registers
@@ -396,9 +398,9 @@ namespace
         if(blockWide)
         {
             const u32 key = freeOff == kNullOffset ? 255u : (h.pageOf(freeOff) & 255u);
-            partitionByKey<u64, 256>(key, freeOff, freeOff, slot);
+            partitionByKey<u64, B>(key, freeOff, freeOff, slot);
             slot = threadIdx.x;
-            partitionByKey<u32, 256>(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
+            partitionByKey<u32, B>(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
         }
         wa.free(freeOff == kNullOffset ? nullptr : h.pages + freeOff);
         void* p = wa.malloc<Dist>(bytes);
@@ -519,6 +521,19 @@ namespace
                 }
             }
         }
+    }
+
+    template<u32 B>
+    void launchMultiBlock(bool xm, const ScatterHeap* heaps, const mmc_step_desc& d, const WaveTuning& t,
+                          bool part, hipStream_t s)
+    {
+        const unsigned blocks = (d.n_tiles + B / 64u - 1u) / (B / 64u);
+        if(xm)
+            hipLaunchKernelGGL((multiStepBlockKernel<XMallocSIMD64, B>), dim3(blocks), dim3(B), 0, s, heaps, d, t,
+                               part);
+        else
+            hipLaunchKernelGGL((multiStepBlockKernel<NoopDistribution, B>), dim3(blocks), dim3(B), 0, s, heaps, d,
+                               t, part);
     }
 
     unsigned persistentBlocks(const mmc_tuning* t)
@@ -716,14 +731,19 @@ int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tu
     }
     else
     {
-        const unsigned blocks = (desc->n_tiles + 3u) / 4u;
         const bool part = partitionOf(tuning);
-        if(xm)
-            hipLaunchKernelGGL(multiStepBlockKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc,
-                               t, part);
-        else
-            hipLaunchKernelGGL(multiStepBlockKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, m->dHeaps,
-                               *desc, t, part);
+        switch(blockOf(tuning))
+        {
+        case 1024u:
+            launchMultiBlock<1024>(xm, m->dHeaps, *desc, t, part, s);
+            break;
+        case 512u:
+            launchMultiBlock<512>(xm, m->dHeaps, *desc, t, part, s);
+            break;
+        default:
+            launchMultiBlock<256>(xm, m->dHeaps, *desc, t, part, s);
+            break;
+        }
     }
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "multi step launch");

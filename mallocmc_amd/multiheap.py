@@ -6,8 +6,9 @@ step every live slot frees with probability 1/2 and every empty slot
 allocates with probability 1/2 a size drawn from Zipf(1.2) over the powers of
 two 16..65536 B (the steady state keeps about half the slots live).  100
 steps, each one graph replay of mmc_multi_step.  The autotuner sweeps
-pagesize {4K, 16K, 64K} x regionsize {8, 16, 32} x aggregation {1, 16, 64} x
-launch form {persistent waves pulling tiles, one workgroup per four tiles}.
+pagesize {4K, 16K, 64K} x regionsize {8, 16, 32} x launch form {persistent
+waves pulling tiles, one workgroup of 256, 512 or 1024 threads per 4, 8 or 16
+tiles} at aggregation 64, then aggregation {1, 16} at the best point.
 """
 from __future__ import annotations
 
@@ -139,28 +140,50 @@ class MultiRun:
         return ok
 
 
+#: launch forms of the step: (persistent, threads per workgroup of the non-persistent form)
+LAUNCH_FORMS = ((True, 0), (False, 256), (False, 512), (False, 1024))
+
+
+def _tuning(p: dict) -> mmc.Tuning:
+    return mmc.Tuning(aggregation=p["aggregation"], persistent=p["persistent"], block_threads=p["block_threads"],
+                      xcd_slices=8)
+
+
 def run_default(log=None, steps: int = 100, tune_steps: int = 20, space=None) -> dict:
+    """Autotunes in two stages (pagesize x regionsize x launch form at the
+    widest aggregation, then the other aggregations at the best point), then
+    runs `steps` graph-replayed steps at the best point and the reference
+    geometry."""
     work = make_workload()
     space = space or dict(pagesize=(4096, 16384, 65536), regionsize=(8, 16, 32), aggregation=(1, 16, 64),
-                          persistent=(True, False))
+                          launch=LAUNCH_FORMS)
     points = []
     t0 = time.time()
+
+    def measure(P, R, agg, persistent, block_threads):
+        point = dict(pagesize=P, regionsize=R, aggregation=agg, persistent=persistent, block_threads=block_threads)
+        run = MultiRun(work, P, R)
+        tun = _tuning(point)
+        run.run(3, tun)  # warm up into the steady state
+        ms, dc = run.run(tune_steps, tun)
+        run.close()
+        point.update(ops_per_s=float((dc[0] + dc[1]) / (ms / 1e3)), null_frac=float(dc[2] / max(1, dc[0])))
+        points.append(point)
+        if log:
+            log(f"multi tune {point}")
+
+    widest = max(space["aggregation"])
     for P, R in itertools.product(space["pagesize"], space["regionsize"]):
-        for agg, persistent in itertools.product(space["aggregation"], space["persistent"]):
-            run = MultiRun(work, P, R)
-            tun = mmc.Tuning(aggregation=agg, persistent=persistent, xcd_slices=8)
-            run.run(3, tun)  # warm up into the steady state
-            ms, dc = run.run(tune_steps, tun)
-            rate = (dc[0] + dc[1]) / (ms / 1e3)
-            points.append(dict(pagesize=P, regionsize=R, aggregation=agg, persistent=persistent,
-                               ops_per_s=float(rate), null_frac=float(dc[2] / max(1, dc[0]))))
-            run.close()
-            if log:
-                log(f"multi tune {points[-1]}")
+        for persistent, bt in space["launch"]:
+            measure(P, R, widest, persistent, bt)
+    b = max(points, key=lambda p: p["ops_per_s"])
+    for agg in space["aggregation"]:
+        if agg != widest:
+            measure(b["pagesize"], b["regionsize"], agg, b["persistent"], b["block_threads"])
     best = max(points, key=lambda p: p["ops_per_s"])
     tune_s = time.time() - t0
     run = MultiRun(work, best["pagesize"], best["regionsize"])
-    tun = mmc.Tuning(aggregation=best["aggregation"], persistent=best["persistent"], xcd_slices=8)
+    tun = _tuning(best)
     ms, dc = run.run(steps, tun)
     ok = run.check_invariants()
     run.close()

@@ -274,19 +274,22 @@ def _check_live(torch, allocs, dev, slot_heap):
         assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0, r
 
 
-@pytest.mark.parametrize("persistent", [True, False])
-def test_multi_heap_steps_and_graph_replay(mmc, cuda, persistent):
+@pytest.mark.parametrize("persistent,block_threads,per",
+                         [(True, 0, 4096), (False, 256, 4096), (False, 1024, 4096), (False, 1024, 1500)])
+def test_multi_heap_steps_and_graph_replay(mmc, cuda, persistent, block_threads, per):
     """8 heaps, 20 steps eager, then 20 steps replayed from a captured graph;
     the live set follows the seeded coins exactly and free-all restores every
     heap's slot count.  Both launch forms: persistent waves pulling tiles, and
-    one workgroup per four tiles (which reorders a one-heap workgroup's
-    requests by page and size)."""
+    one workgroup per 4 or 16 tiles, which reorders a one-heap workgroup's
+    requests by page and size.  With 4096 slots per heap every workgroup lies
+    in one heap; with 1500 (24 tiles, the last one ragged) workgroups of 16
+    tiles straddle two heaps and serve them unpartitioned."""
     torch = cuda
-    nheaps, per = 8, 4096
+    nheaps = 8
     allocs, multi, dev, d, slot_heap = _step_setup(torch, mmc, nheaps, per, 64 * MiB, 16384)
     s0 = [a.getAvailableSlots(64) for a in allocs]
-    tun = mmc.Tuning(persistent=persistent, aggregation=64)
-    print(f"multi-heap steps persistent={persistent}", flush=True)
+    tun = mmc.Tuning(persistent=persistent, aggregation=64, block_threads=block_threads)
+    print(f"multi-heap steps persistent={persistent} block_threads={block_threads}", flush=True)
     live_hist = []
     for _ in range(20):
         multi.step(d, tun)
