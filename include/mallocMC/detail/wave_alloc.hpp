@@ -594,8 +594,10 @@ namespace mallocMC
                         {
                             const u32 region = wrap(region0 + from + skipped + regRel[l]);
                             const u32 page = region * R + (pageRel - regRel[l] * R);
-                            css[l] = atom::load(h.pteChunksize(page));
-                            cnts[l] = atom::load(h.pteCount(page));
+                            // chunk size and count in one load of their paired word
+                            const u64 pr = atom::load(h.ptePair(page));
+                            css[l] = static_cast<u32>(pr);
+                            cnts[l] = static_cast<u32>(pr >> 32);
                         }
                     }
                     const u64 openMask = hw::ballot(me < nregs && fill < checklevel);
@@ -711,8 +713,9 @@ namespace mallocMC
                             if(valid)
                             {
                                 fill = atom::load(&h.regions[region]);
-                                pcs = atom::load(h.pteChunksize(region * R + pj));
-                                pcount = atom::load(h.pteCount(region * R + pj));
+                                const u64 pr = atom::load(h.ptePair(region * R + pj));
+                                pcs = static_cast<u32>(pr);
+                                pcount = static_cast<u32>(pr >> 32);
                                 if(flat)
                                     pbits = atom::load(h.pteBitmask(region * R + pj));
                             }
