@@ -82,15 +82,18 @@ def test_snapshot_payload_holds_the_pattern(mmc, cuda):
     a.close()
 
 
-def test_snapshot_after_multi_heap_steps(mmc, cuda):
-    """Concurrent mallocs and frees of the multi-heap step leave every heap
-    consistent with the slots' live allocations."""
+@pytest.mark.parametrize("persistent,block_threads", [(True, 0), (False, 512)])
+def test_snapshot_after_multi_heap_steps(mmc, cuda, persistent, block_threads):
+    """Concurrent mallocs and frees of the multi-heap step (persistent, or one
+    partitioning workgroup per 8 tiles) leave every heap consistent with the
+    slots' live allocations."""
     from mallocmc_amd import snapshot as S
     from test_gpu_wave import _step_setup
     torch = cuda
     nheaps, per = 4, 8192
     allocs, multi, dev, d, slot_heap = _step_setup(torch, mmc, nheaps, per, 128 * MiB, 4096, seed=13, nsizes=13)
-    tun = mmc.Tuning(persistent=True, aggregation=64)
+    tun = mmc.Tuning(persistent=persistent, aggregation=64, block_threads=block_threads)
+    print(f"snapshot after multi-heap steps persistent={persistent} block_threads={block_threads}", flush=True)
     for _ in range(10):
         multi.step(d, tun)
     torch.cuda.synchronize()
