@@ -350,6 +350,12 @@ namespace
         __shared__ u32 sHeap[W];
         __shared__ u32 sFirst[W];
         __shared__ u32 sLast;
+        // the workgroup's mallocs, frees, nulls and tiles, added to the
+        // global counters once per workgroup: atomics on one address are
+        // performed one after another, and one per wave on the same four
+        // words (tens of thousands of waves per step) took longer than the
+        // step's own work on nearly empty heaps
+        __shared__ unsigned long long sCount[4];
         const u64 step = *d.d_step;
         const u32 me = hw::laneid();
         const u32 w = threadIdx.x / 64u;
@@ -362,6 +368,8 @@ namespace
             sHeap[w] = heapIdx;
             sFirst[w] = first;
         }
+        if(threadIdx.x < 4u)
+            sCount[threadIdx.x] = 0u;
         __syncthreads();
         bool blockWide = partition && sHeap[0] != 0xffffffffu;
         for(u32 i = 1; i < W; ++i)
@@ -423,17 +431,19 @@ namespace
         if(me == 0u)
         {
             if(nMalloc)
-                atom::add(&d.d_counters[0], nMalloc);
+                atomicAdd(&sCount[0], static_cast<unsigned long long>(nMalloc));
             if(nFree)
-                atom::add(&d.d_counters[1], nFree);
+                atomicAdd(&sCount[1], static_cast<unsigned long long>(nFree));
             if(nNull)
-                atom::add(&d.d_counters[2], nNull);
+                atomicAdd(&sCount[2], static_cast<unsigned long long>(nNull));
             if(haveTile)
-                atom::add(&d.d_counters[3], u64{1});
+                atomicAdd(&sCount[3], 1ull);
         }
         // the last workgroup to finish advances the step (every workgroup read
         // it at its start) and resets the finish counter for the next launch
         __syncthreads();
+        if(threadIdx.x < 4u && sCount[threadIdx.x] != 0u)
+            atom::add(&d.d_counters[threadIdx.x], static_cast<u64>(sCount[threadIdx.x]));
         if(threadIdx.x == 0u)
             sLast = atom::add(&d.d_tile_cursor[1], 1u) == gridDim.x - 1u ? 1u : 0u;
         __syncthreads();
