@@ -32,11 +32,13 @@ namespace
         u32 cursor;
         u32 freeTop;
         u32 nDone;
-        u32 ticket;   // completion blocks finished this step (self-resetting)
+        u32 nChunks;  // checksum chunks of the requests finished this step (decode adds them)
         u64 counters[cCount];
         u32 nStamps;  // device-clock stamps recorded since load
-        u32 pad;
+        u32 chunksFinished; // chunks whose checksum (and release, for a last chunk) is done
     };
+
+    constexpr u32 kTokensPerChunk = 64u;
 
     //! Everything a serving kernel needs, passed by value.
     struct ServeView
@@ -226,7 +228,10 @@ namespace
             const u32 t1 = t + 1u;
             v.slotCur[slot] = t1;
             if(t1 == len)
+            {
                 v.doneList[atom::add(&v.sc->nDone, 1u)] = slot;
+                atom::add(&v.sc->nChunks, divup(len, kTokensPerChunk));
+            }
         }
         u32 mallocs = needBlock ? 1u : 0u;
         for(int d = 32; d > 0; d >>= 1)
@@ -243,7 +248,6 @@ namespace
         }
     }
 
-    constexpr u32 kTokensPerChunk = 64u;
     constexpr u32 kCompleteBlocks = 1024u;
 
     //! Returns a finished request's blocks (wave-cooperative free), slot and
@@ -282,19 +286,38 @@ namespace
     //! cut into 64-token chunks spread over the whole grid, are read back into
     //! their checksums (lane = word of a token, the 4 waves of a workgroup take
     //! every fourth token, all 16 of a wave's tokens in flight at once;
-    //! partial sums add atomically).  The workgroup that finishes a request's last chunk
-    //! releases the request; the last workgroup to finish resets the done list
-    //! and admits the next step's arrivals, so one step is two launches.
+    //! partial sums add atomically).  The workgroup that finishes a request's
+    //! last chunk releases the request.  The workgroup that finishes the
+    //! step's last chunk (a count of finished chunks against the total that
+    //! decode added up) resets the done list and admits the next step's
+    //! arrivals, so one step is two launches.  Workgroups without a chunk
+    //! leave at once and touch no shared word: a ticket taken by all 1024
+    //! workgroups of the grid, on one address, cost about 7 us of the step.
     __global__ __launch_bounds__(256) void completeKernel(ServeView v)
     {
         __shared__ u64 partial[4];
         __shared__ u32 lastChunk;
+        __shared__ u32 admit;
         const u32 lane = hw::laneid();
         const u32 q = threadIdx.x / 64u;
         const u32 nDone = atom::load(&v.sc->nDone);
+        const u32 total = atom::load(&v.sc->nChunks);
+        if(blockIdx.x >= total)
+        {
+            // no chunk here; with none at all, the first workgroup admits
+            if(total == 0u && blockIdx.x == 0u && q == 0u)
+            {
+                if(lane == 0u)
+                    atom::store(&v.sc->nDone, 0u);
+                admitWave(v);
+            }
+            return;
+        }
+        if(threadIdx.x == 0u)
+            admit = 0u;
         const u32 wpt = v.tokenBytes / 4u;
         u32 i = 0u, base = 0u; // done-list entry and its first chunk id
-        for(u32 c = blockIdx.x;; c += gridDim.x)
+        for(u32 c = blockIdx.x; c < total; c += gridDim.x)
         {
             // the entry holding chunk c: 64 entries per round trip (lane k
             // reads entry i + k and its length), a prefix sum of their chunk
@@ -377,18 +400,20 @@ namespace
             __syncthreads();
             if(lastChunk && q == 0u)
                 release(v, slot, len);
+            // counted after the chunk's release (thread 0 is in the wave that
+            // released, and release waits for the results admission reads)
+            if(threadIdx.x == 0u && atom::add(&v.sc->chunksFinished, 1u) + 1u == total)
+                admit = 1u;
             __syncthreads();
         }
-        __shared__ u32 last;
-        if(threadIdx.x == 0u)
-            last = atom::add(&v.sc->ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
         __syncthreads();
-        if(last && q == 0u)
+        if(admit && q == 0u)
         {
-            // last workgroup: every release of this step is done
+            // the step's last chunk: every release of this step is done
             if(lane == 0u)
             {
-                atom::store(&v.sc->ticket, 0u);
+                atom::store(&v.sc->chunksFinished, 0u);
+                atom::store(&v.sc->nChunks, 0u);
                 atom::store(&v.sc->nDone, 0u);
             }
             admitWave(v);
@@ -412,7 +437,8 @@ namespace
             v.sc->cursor = 0u;
             v.sc->freeTop = v.maxSlots;
             v.sc->nDone = 0u;
-            v.sc->ticket = 0u;
+            v.sc->nChunks = 0u;
+            v.sc->chunksFinished = 0u;
             v.sc->nStamps = 0u;
             for(u32 c = 0; c < cCount; ++c)
                 v.sc->counters[c] = 0u;
