@@ -201,9 +201,10 @@ def test_wave_multipage_fill_to_null(mmc, cuda):
     a.close()
 
 
-def _step_setup(torch, mmc, nheaps, slots_per_heap, heap_bytes, pagesize, seed=7, nsizes=8):
+def _step_setup(torch, mmc, nheaps, slots_per_heap, heap_bytes, pagesize, seed=7, nsizes=8, distribution=None):
     cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=heap_bytes // 4, regionsize=16,
-                            resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
+                            resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK,
+                            distribution=mmc.DIST_NOOP if distribution is None else distribution)
     allocs = [mmc.ScatterAllocator(cfg, heap_bytes) for _ in range(nheaps)]
     multi = mmc.MultiHeap(allocs)
     slot_heap = np.repeat(np.arange(nheaps, dtype=np.uint32), slots_per_heap)
@@ -368,3 +369,32 @@ def test_wave_primitives_selftest(mmc, cuda):
     assert rc == 0
     torch.cuda.synchronize()
     assert res.cpu().tolist() == [0, 0, 0, 0]
+
+
+@pytest.mark.parametrize("persistent,block_threads", [(True, 0), (False, 1024)])
+def test_multi_heap_steps_xmallocsimd(mmc, cuda, persistent, block_threads):
+    """Multi-heap steps over XMallocSIMD heaps: a wave's small requests of
+    one size class share one allocation (XMallocSIMD collect/distribute),
+    also after the workgroup form reordered them by size.  Live allocations
+    never overlap, patterns survive, and free-all restores every heap."""
+    torch = cuda
+    nheaps, per = 4, 4096
+    allocs, multi, dev, d, slot_heap = _step_setup(torch, mmc, nheaps, per, 64 * MiB, 16384, seed=17,
+                                                   distribution=mmc.DIST_XMALLOCSIMD)
+    s0 = [a.getAvailableSlots(64) for a in allocs]
+    tun = mmc.Tuning(persistent=persistent, aggregation=64, block_threads=block_threads)
+    print(f"xmallocsimd multi-heap steps persistent={persistent} block_threads={block_threads}", flush=True)
+    for _ in range(10):
+        multi.step(d, tun)
+    torch.cuda.synchronize()
+    c = dev["counters"].cpu().numpy()
+    assert c[0] > 0 and c[1] > 0 and c[2] == 0
+    _check_live(torch, allocs, dev, slot_heap)
+    for hidx, a in enumerate(allocs):
+        sel = torch.from_numpy(np.nonzero(slot_heap == hidx)[0]).cuda()
+        a.wave_free_batch(dev["offs"][sel].contiguous())
+    torch.cuda.synchronize()
+    assert [a.getAvailableSlots(64) for a in allocs] == s0
+    multi.close()
+    for a in allocs:
+        a.close()
