@@ -196,8 +196,9 @@ def main():
         tuned_points = 0
     else:
         t0 = time.time()
-        # persistent launches never win on this one-heap workload (they do on
-        # the multi-heap step); the batched form sweeps the rest
+        # persistent launches never win on this one-heap workload: they serve
+        # requests in arrival order, without the block partition (three to
+        # four times slower); the batched form sweeps the rest
         space = dict(autotune.DEFAULT_SPACE, persistent=(False,))
         best, points = autotune.tune(base, HEAP, batches, space=space, log=log)
         tuned_points = len(points)
