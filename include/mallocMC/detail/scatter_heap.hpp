@@ -736,13 +736,6 @@ namespace mallocMC
             }
 
             // ------------------------------------------ slot counting
-            //! countFreeChunksInPage (Scatter.hpp:1112-1139)
-            MALLOCMC_D u32 countFreeChunksInPage(u32 page, u32 chunksize) const
-            {
-                const u32 filled = atom::load(pteCount(page));
-                return pageCapacity(chunksize) - filled;
-            }
-
             //! getAvailaibleSlotsDeviceFunction (Scatter.hpp:1158-1219)
             MALLOCMC_D u32 availableSlotsPartial(u64 slotSize, u32 gid, u32 stride) const
             {
@@ -752,14 +745,17 @@ namespace mallocMC
                     const u32 maxchunksize = umin32(pagesize, wastefactor * static_cast<u32>(slotSize));
                     for(u32 p = gid; p < numpages; p += stride)
                     {
-                        u32 cs = atom::load(pteChunksize(p));
+                        // chunk size and count in one load of their paired
+                        // word; free chunks = capacity - count
+                        // (countFreeChunksInPage, Scatter.hpp:1112-1139,
+                        // reads the count after the chunk size)
+                        const u64 pr = atom::load(ptePair(p));
+                        const u32 cs = static_cast<u32>(pr);
+                        const u32 filled = static_cast<u32>(pr >> 32);
                         if(cs >= slotSize && cs <= maxchunksize)
-                            slots += countFreeChunksInPage(p, cs);
+                            slots += pageCapacity(cs) - filled;
                         else if(cs == 0u)
-                        {
-                            cs = umax32(static_cast<u32>(slotSize), paddedMinChunk);
-                            slots += countFreeChunksInPage(p, cs);
-                        }
+                            slots += pageCapacity(umax32(static_cast<u32>(slotSize), paddedMinChunk)) - filled;
                     }
                 }
                 else

@@ -106,16 +106,32 @@ namespace
     }
 
     //! getAvailableSlotsHost's kernel (Scatter.hpp:1245-1257) for slot sizes
-    //! below a page: page-strided partial counts, one atomic per wave.
-    __global__ void slotsKernel(ScatterHeap h, u64 slotSize, u32* out)
+    //! below a page: page-strided partial counts, summed per workgroup in LDS
+    //! and added with one atomic per workgroup (atomics on one address are
+    //! performed one after another: one per wave made this launch several
+    //! times longer than its PTE reads).
+    constexpr unsigned kSlotsThreads = 1024u;
+    constexpr unsigned kSlotsMaxBlocks = 512u;
+
+    __global__ __launch_bounds__(kSlotsThreads) void slotsKernel(ScatterHeap h, u64 slotSize, u32* out)
     {
+        __shared__ u32 waveSum[kSlotsThreads / 64u];
         const u32 gid = blockIdx.x * blockDim.x + threadIdx.x;
         const u32 stride = gridDim.x * blockDim.x;
         u32 v = h.availableSlotsPartial(slotSize, gid, stride);
         for(int d = 32; d > 0; d >>= 1)
             v += __shfl_xor(v, d, 64);
-        if(hw::laneid() == 0 && v != 0u)
-            atom::add(out, v);
+        if(hw::laneid() == 0u)
+            waveSum[threadIdx.x / 64u] = v;
+        __syncthreads();
+        if(threadIdx.x == 0u)
+        {
+            u32 total = 0u;
+            for(u32 w = 0; w < kSlotsThreads / 64u; ++w)
+                total += waveSum[w];
+            if(total != 0u)
+                atom::add(out, total);
+        }
     }
 
     //! Slot sizes of a page or more: the reference counts, from the top page
@@ -359,10 +375,10 @@ namespace mmc
         const ScatterHeap& h = heap->view;
         if(padded < h.pagesize)
         {
-            const unsigned threads = 256;
-            u64 blocks = (h.numpages + threads - 1u) / threads;
-            blocks = blocks < 2048u ? (blocks ? blocks : 1u) : 2048u;
-            hipLaunchKernelGGL(slotsKernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0, stream, h, padded, dOut);
+            u64 blocks = (h.numpages + kSlotsThreads - 1u) / kSlotsThreads;
+            blocks = blocks < kSlotsMaxBlocks ? (blocks ? blocks : 1u) : kSlotsMaxBlocks;
+            hipLaunchKernelGGL(slotsKernel, dim3(static_cast<unsigned>(blocks)), dim3(kSlotsThreads), 0, stream, h,
+                               padded, dOut);
         }
         else
         {
