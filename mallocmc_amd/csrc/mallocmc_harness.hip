@@ -32,7 +32,7 @@ namespace
         }
     }
 
-    __global__ void verifyPatternKernel(ScatterHeap h, const u64* offsets, const u32* sizes, u64 n, u64 salt,
+    __global__ __launch_bounds__(256) void verifyPatternKernel(ScatterHeap h, const u64* offsets, const u32* sizes, u64 n, u64 salt,
                                         u64* result)
     {
         const u64 wave = (static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x) / 64u;
@@ -75,18 +75,24 @@ namespace
             nulls += __shfl_xor(nulls, d, 64);
             checked += __shfl_xor(checked, d, 64);
         }
+        // workgroup totals first: one atomic per counter and workgroup, not
+        // per wave (atomics on one address are performed one after another)
+        __shared__ u64 sums[5][4];
+        const u32 w = threadIdx.x / 64u;
         if(lane == 0)
         {
-            if(bad)
-                atom::add(&result[0], bad);
-            if(misaligned)
-                atom::add(&result[1], misaligned);
-            if(outside)
-                atom::add(&result[2], outside);
-            if(nulls)
-                atom::add(&result[3], nulls);
-            if(checked)
-                atom::add(&result[4], checked);
+            sums[0][w] = bad;
+            sums[1][w] = misaligned;
+            sums[2][w] = outside;
+            sums[3][w] = nulls;
+            sums[4][w] = checked;
+        }
+        __syncthreads();
+        if(threadIdx.x < 5u)
+        {
+            const u64 t = sums[threadIdx.x][0] + sums[threadIdx.x][1] + sums[threadIdx.x][2] + sums[threadIdx.x][3];
+            if(t)
+                atom::add(&result[threadIdx.x], t);
         }
     }
 
@@ -215,9 +221,9 @@ int mmc_verify_pattern(mmc_heap* heap, const uint64_t* d_offsets, const uint32_t
         return mmc::hipFail(err, "hipMemsetAsync");
     if(n == 0)
         return MMC_OK;
-    const unsigned threads = 256;
+    const unsigned threads = 256; // four waves: the kernel's workgroup totals assume it
     u64 blocks = (n * 64u + threads - 1u) / threads;
-    blocks = blocks < 16384u ? blocks : 16384u;
+    blocks = blocks < 2048u ? blocks : 2048u;
     hipLaunchKernelGGL(verifyPatternKernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0, s, heap->view,
                        d_offsets, d_sizes, n, salt, d_result);
     err = hipGetLastError();
