@@ -198,6 +198,16 @@ namespace
             if(!needBlock)
                 off = blocks[t / v.tokensPerBlock];
         }
+        // a request that reaches its length this step takes its done-list
+        // entry now: the returning atomic travels with the malloc's loads
+        // instead of adding a round trip after them
+        const bool finishing = active && t + 1u == len;
+        u32 doneAt = 0u;
+        if(finishing)
+        {
+            doneAt = atom::add(&v.sc->nDone, 1u);
+            atom::add(&v.sc->nChunks, divup(len, kTokensPerChunk));
+        }
         const WaveAlloc wa{v.heap, WaveTuning{}};
         void* p = wa.malloc<NoopDistribution>(needBlock ? v.blockBytes : 0u);
         u32 nulls = 0u;
@@ -225,13 +235,9 @@ namespace
                     *reinterpret_cast<uint4*>(dst + w) = q;
                 }
             }
-            const u32 t1 = t + 1u;
-            v.slotCur[slot] = t1;
-            if(t1 == len)
-            {
-                v.doneList[atom::add(&v.sc->nDone, 1u)] = slot;
-                atom::add(&v.sc->nChunks, divup(len, kTokensPerChunk));
-            }
+            v.slotCur[slot] = t + 1u;
+            if(finishing)
+                v.doneList[doneAt] = slot;
         }
         u32 mallocs = needBlock ? 1u : 0u;
         for(int d = 32; d > 0; d >>= 1)
