@@ -51,7 +51,9 @@ namespace
         u32* slotCur;
         u32* chunkDone; // checksum chunks finished per slot (self-resetting)
         u64* slotBlocks;
-        u32* doneList;
+        u32* doneList; // slots of the requests finished this step, in decode's order
+        u32* doneLen;  // their lengths and request indices, so that completion
+        u32* doneReq;  // reads an entry in one round trip
         const u32* arrival;
         const u32* length;
         const u32* ids;
@@ -243,7 +245,11 @@ namespace
             }
             v.slotCur[slot] = t + 1u;
             if(finishing)
+            {
                 v.doneList[doneAt] = slot;
+                v.doneLen[doneAt] = len;
+                v.doneReq[doneAt] = req;
+            }
         }
         u32 mallocs = needBlock ? 1u : 0u;
         for(int d = 32; d > 0; d >>= 1)
@@ -332,15 +338,17 @@ namespace
         for(u32 c = blockIdx.x; c < total; c += gridDim.x)
         {
             // the entry holding chunk c: 64 entries per round trip (lane k
-            // reads entry i + k and its length), a prefix sum of their chunk
-            // counts and a ballot, instead of one dependent load pair per entry
-            u32 nChunks = 0u, slot = 0u, len = 0u;
+            // reads entry i + k: slot, length and request side by side), a
+            // prefix sum of their chunk counts and a ballot, instead of one
+            // dependent load pair per entry
+            u32 nChunks = 0u, slot = 0u, len = 0u, req = 0u;
             bool found = false;
             while(i < nDone)
             {
                 const u32 e = i + lane;
                 const u32 es = e < nDone ? v.doneList[e] : 0u;
-                const u32 el = e < nDone ? v.slotLen[es] : 0u;
+                const u32 el = e < nDone ? v.doneLen[e] : 0u;
+                const u32 er = e < nDone ? v.doneReq[e] : 0u;
                 const u32 n = divup(el, kTokensPerChunk);
                 u32 incl = n;
                 for(u32 d = 1u; d < 64u; d <<= 1)
@@ -356,6 +364,7 @@ namespace
                     nChunks = wave::lane(n, k);
                     slot = wave::lane(es, k);
                     len = wave::lane(el, k);
+                    req = wave::lane(er, k);
                     base += wave::lane(incl, k) - nChunks;
                     i += k;
                     found = true;
@@ -366,7 +375,6 @@ namespace
             }
             if(!found)
                 break;
-            const u32 req = v.slotReq[slot];
             const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
             const u32 t0 = (c - base) * kTokensPerChunk;
             const u32 t1 = umin32(len, t0 + kTokensPerChunk);
@@ -497,7 +505,7 @@ int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** ou
     s->cfg = *cfg;
     const u32 maxBlocks = divup(cfg->max_tokens, cfg->tokens_per_block);
     const u64 n = cfg->max_slots;
-    const u64 bytes = sizeof(ServeScalars) + 4u * n * 6u + 8u * n * maxBlocks + 256u;
+    const u64 bytes = sizeof(ServeScalars) + 4u * n * 8u + 8u * n * maxBlocks + 256u;
     hipError_t err = hipMalloc(&s->arena, bytes);
     if(err != hipSuccess)
     {
@@ -520,6 +528,10 @@ int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** ou
     v.slotCur = reinterpret_cast<u32*>(a);
     a += 4u * n;
     v.doneList = reinterpret_cast<u32*>(a);
+    a += 4u * n;
+    v.doneLen = reinterpret_cast<u32*>(a);
+    a += 4u * n;
+    v.doneReq = reinterpret_cast<u32*>(a);
     a += 4u * n;
     v.chunkDone = reinterpret_cast<u32*>(a);
     v.maxSlots = cfg->max_slots;
