@@ -35,7 +35,7 @@ namespace
         u32 nChunks;  // checksum chunks of the requests finished this step (decode adds them)
         u64 counters[cCount];
         u32 nStamps;  // device-clock stamps recorded since load
-        u32 chunksFinished; // chunks whose checksum (and release, for a last chunk) is done
+        u32 released; // requests of this step's done list released so far
     };
 
     constexpr u32 kTokensPerChunk = 64u;
@@ -305,12 +305,14 @@ namespace
     //! their checksums (lane = word of a token, the 4 waves of a workgroup take
     //! every fourth token, all 16 of a wave's tokens in flight at once;
     //! partial sums add atomically).  The workgroup that finishes a request's
-    //! last chunk releases the request.  The workgroup that finishes the
-    //! step's last chunk (a count of finished chunks against the total that
-    //! decode added up) resets the done list and admits the next step's
-    //! arrivals, so one step is two launches.  Workgroups without a chunk
-    //! leave at once and touch no shared word: a ticket taken by all 1024
-    //! workgroups of the grid, on one address, cost about 7 us of the step.
+    //! last chunk releases the request.  The workgroup that releases the
+    //! step's last request (a count of releases against the done list's
+    //! length) resets the done list and admits the next step's arrivals, so
+    //! one step is two launches.  Workgroups without a chunk (decode added up
+    //! the step's chunk count) leave at once and touch no shared word: a
+    //! ticket taken by all 1024 workgroups of the grid, on one address, cost
+    //! about 7 us of the step, and one count per chunk instead of per request
+    //! put a few hundred atomics on one word.
     __global__ __launch_bounds__(256) void completeKernel(ServeView v)
     {
         __shared__ u64 partial[4];
@@ -419,11 +421,13 @@ namespace
             }
             __syncthreads();
             if(lastChunk && q == 0u)
+            {
                 release(v, slot, len);
-            // counted after the chunk's release (thread 0 is in the wave that
-            // released, and release waits for the results admission reads)
-            if(threadIdx.x == 0u && atom::add(&v.sc->chunksFinished, 1u) + 1u == total)
-                admit = 1u;
+                // counted after the release (which waits for the results
+                // admission reads); one count per request, not per chunk
+                if(lane == 0u && atom::add(&v.sc->released, 1u) + 1u == nDone)
+                    admit = 1u;
+            }
             __syncthreads();
         }
         __syncthreads();
@@ -432,7 +436,7 @@ namespace
             // the step's last chunk: every release of this step is done
             if(lane == 0u)
             {
-                atom::store(&v.sc->chunksFinished, 0u);
+                atom::store(&v.sc->released, 0u);
                 atom::store(&v.sc->nChunks, 0u);
                 atom::store(&v.sc->nDone, 0u);
             }
@@ -458,7 +462,7 @@ namespace
             v.sc->freeTop = v.maxSlots;
             v.sc->nDone = 0u;
             v.sc->nChunks = 0u;
-            v.sc->chunksFinished = 0u;
+            v.sc->released = 0u;
             v.sc->nStamps = 0u;
             for(u32 c = 0; c < cCount; ++c)
                 v.sc->counters[c] = 0u;
