@@ -15,9 +15,9 @@
  *   3. completion: a request that reached its length reads its whole buffer
  *      back into a 64-bit checksum, frees its blocks (wave-cooperative free)
  *      and returns its slot and reservation.
- * A step is two launches: decode, then completion, where the workgroup
- * that finishes the step's last checksum chunk also runs the admission for
- * the next step (the first admission runs at load).  Both can be captured in one graph; the step counter lives in
+ * A step is two launches: decode, then completion, where the wave that
+ * makes the step's last release also runs the admission for the next step
+ * (the first admission runs at load).  Both can be captured in one graph; the step counter lives in
  * device memory.  Payload word w of token t of request r is
  * mmc_serve_word(r, t, w) below, so every checksum has a closed form.
  */

@@ -49,7 +49,6 @@ namespace
         u32* slotReq;
         u32* slotLen;
         u32* slotCur;
-        u32* chunkDone; // checksum chunks finished per slot (self-resetting)
         u64* slotBlocks;
         u32* doneList; // slots of the requests finished this step, in decode's order
         u32* doneLen;  // their lengths and request indices, so that completion
@@ -270,7 +269,7 @@ namespace
 
     //! Returns a finished request's blocks (wave-cooperative free), slot and
     //! reservation; the whole wave calls it.  The results that the admission
-    //! at the end of the launch reads are returning atomics, waited for here.
+    //! reads are returning atomics, waited for here.
     __device__ void release(const ServeView& v, u32 slot, u32 len)
     {
         const u32 lane = hw::laneid();
@@ -293,7 +292,6 @@ namespace
             atom::add(&v.sc->counters[cFrees], static_cast<u64>(freed));
             const u64 r = atom::sub(&v.sc->reserved, static_cast<u64>(nb) * v.blockBytes);
             const u32 q = atom::exch(&v.slotReq[slot], kEmpty);
-            atom::store(&v.chunkDone[slot], 0u);
             const u32 pos = atom::add(&v.sc->freeTop, 1u);
             const u32 e = atom::exch(&v.freeStack[pos], slot);
             hw::consume(static_cast<u32>(r) ^ q ^ e);
@@ -304,20 +302,18 @@ namespace
     //! cut into 64-token chunks spread over the whole grid, are read back into
     //! their checksums (lane = word of a token, the 4 waves of a workgroup take
     //! every fourth token, all 16 of a wave's tokens in flight at once;
-    //! partial sums add atomically).  The workgroup that finishes a request's
-    //! last chunk releases the request.  The workgroup that releases the
-    //! step's last request (a count of releases against the done list's
-    //! length) resets the done list and admits the next step's arrivals, so
-    //! one step is two launches.  Workgroups without a chunk (decode added up
-    //! the step's chunk count) leave at once and touch no shared word: a
-    //! ticket taken by all 1024 workgroups of the grid, on one address, cost
-    //! about 7 us of the step, and one count per chunk instead of per request
-    //! put a few hundred atomics on one word.
+    //! partial sums add atomically).  The workgroup holding a request's first
+    //! chunk releases the request at once, beside the reads.  The wave that
+    //! makes the step's last release (a count of releases against the done
+    //! list's length) resets the done list and admits the next step's
+    //! arrivals, so one step is two launches.  Workgroups without a chunk
+    //! (decode added up the step's chunk count) leave at once and touch no
+    //! shared word: a ticket taken by all 1024 workgroups of the grid, on one
+    //! address, cost about 7 us of the step, and one count per chunk instead
+    //! of per request put a few hundred atomics on one word.
     __global__ __launch_bounds__(256) void completeKernel(ServeView v)
     {
         __shared__ u64 partial[4];
-        __shared__ u32 lastChunk;
-        __shared__ u32 admit;
         const u32 lane = hw::laneid();
         const u32 q = threadIdx.x / 64u;
         const u32 nDone = atom::load(&v.sc->nDone);
@@ -333,8 +329,6 @@ namespace
             }
             return;
         }
-        if(threadIdx.x == 0u)
-            admit = 0u;
         const u32 wpt = v.tokenBytes / 4u;
         u32 i = 0u, base = 0u; // done-list entry and its first chunk id
         for(u32 c = blockIdx.x; c < total; c += gridDim.x)
@@ -378,6 +372,28 @@ namespace
             if(!found)
                 break;
             const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
+            if(c == base && q == 3u)
+            {
+                // the request's first chunk: its last wave releases the
+                // request right away, beside the checksum reads (nothing in
+                // this launch allocates, and a free leaves the payload as it
+                // is, so the reads stay valid); the wave that counts the
+                // step's last release admits the next step's arrivals
+                release(v, slot, len);
+                u32 last = 0u;
+                if(lane == 0u)
+                    last = atom::add(&v.sc->released, 1u) + 1u == nDone ? 1u : 0u;
+                if(__builtin_amdgcn_readfirstlane(last) != 0u)
+                {
+                    if(lane == 0u)
+                    {
+                        atom::store(&v.sc->released, 0u);
+                        atom::store(&v.sc->nChunks, 0u);
+                        atom::store(&v.sc->nDone, 0u);
+                    }
+                    admitWave(v);
+                }
+            }
             const u32 t0 = (c - base) * kTokensPerChunk;
             const u32 t1 = umin32(len, t0 + kTokensPerChunk);
             // wave q reads tokens t0 + q, t0 + q + 4, ...: all block offsets
@@ -413,34 +429,8 @@ namespace
                 partial[q] = cs;
             __syncthreads();
             if(threadIdx.x == 0u)
-            {
                 atom::add(&v.checksums[req], partial[0] + partial[1] + partial[2] + partial[3]);
-                // every read of this chunk is done (the loads returned before
-                // the sums): the request's last chunk may release its blocks
-                lastChunk = atom::add(&v.chunkDone[slot], 1u) + 1u == nChunks ? 1u : 0u;
-            }
             __syncthreads();
-            if(lastChunk && q == 0u)
-            {
-                release(v, slot, len);
-                // counted after the release (which waits for the results
-                // admission reads); one count per request, not per chunk
-                if(lane == 0u && atom::add(&v.sc->released, 1u) + 1u == nDone)
-                    admit = 1u;
-            }
-            __syncthreads();
-        }
-        __syncthreads();
-        if(admit && q == 0u)
-        {
-            // the step's last chunk: every release of this step is done
-            if(lane == 0u)
-            {
-                atom::store(&v.sc->released, 0u);
-                atom::store(&v.sc->nChunks, 0u);
-                atom::store(&v.sc->nDone, 0u);
-            }
-            admitWave(v);
         }
     }
 
@@ -452,7 +442,6 @@ namespace
         {
             v.freeStack[i] = v.maxSlots - 1u - i;
             v.slotReq[i] = kEmpty;
-            v.chunkDone[i] = 0u;
         }
         if(tid == 0u)
         {
@@ -509,7 +498,7 @@ int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** ou
     s->cfg = *cfg;
     const u32 maxBlocks = divup(cfg->max_tokens, cfg->tokens_per_block);
     const u64 n = cfg->max_slots;
-    const u64 bytes = sizeof(ServeScalars) + 4u * n * 8u + 8u * n * maxBlocks + 256u;
+    const u64 bytes = sizeof(ServeScalars) + 4u * n * 7u + 8u * n * maxBlocks + 256u;
     hipError_t err = hipMalloc(&s->arena, bytes);
     if(err != hipSuccess)
     {
@@ -536,8 +525,6 @@ int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** ou
     v.doneLen = reinterpret_cast<u32*>(a);
     a += 4u * n;
     v.doneReq = reinterpret_cast<u32*>(a);
-    a += 4u * n;
-    v.chunkDone = reinterpret_cast<u32*>(a);
     v.maxSlots = cfg->max_slots;
     v.blockBytes = cfg->block_bytes;
     v.tokensPerBlock = cfg->tokens_per_block;
