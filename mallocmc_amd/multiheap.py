@@ -8,7 +8,8 @@ two 16..65536 B (the steady state keeps about half the slots live).  100
 steps, each one graph replay of mmc_multi_step.  The autotuner sweeps
 pagesize {4K, 16K, 64K} x regionsize {8, 16, 32} x launch form {persistent
 waves pulling tiles, one workgroup of 256, 512 or 1024 threads per 4, 8 or 16
-tiles} at aggregation 64, then aggregation {1, 16} at the best point.
+tiles} at aggregation 64 and 8 XCD slices, then aggregation {1, 16, 64} x XCD
+slices {8, 0} at the best point.
 """
 from __future__ import annotations
 
@@ -146,22 +147,23 @@ LAUNCH_FORMS = ((True, 0), (False, 256), (False, 512), (False, 1024))
 
 def _tuning(p: dict) -> mmc.Tuning:
     return mmc.Tuning(aggregation=p["aggregation"], persistent=p["persistent"], block_threads=p["block_threads"],
-                      xcd_slices=8)
+                      xcd_slices=p["xcd_slices"])
 
 
 def run_default(log=None, steps: int = 100, tune_steps: int = 20, space=None) -> dict:
     """Autotunes in two stages (pagesize x regionsize x launch form at the
-    widest aggregation, then the other aggregations at the best point), then
-    runs `steps` graph-replayed steps at the best point and the reference
-    geometry."""
+    widest aggregation and 8 XCD slices, then aggregation x XCD slicing at
+    the best point), then runs `steps` graph-replayed steps at the best point
+    and the reference geometry."""
     work = make_workload()
     space = space or dict(pagesize=(4096, 16384, 65536), regionsize=(8, 16, 32), aggregation=(1, 16, 64),
-                          launch=LAUNCH_FORMS)
+                          launch=LAUNCH_FORMS, xcd_slices=(8, 0))
     points = []
     t0 = time.time()
 
-    def measure(P, R, agg, persistent, block_threads):
-        point = dict(pagesize=P, regionsize=R, aggregation=agg, persistent=persistent, block_threads=block_threads)
+    def measure(P, R, agg, persistent, block_threads, xcd):
+        point = dict(pagesize=P, regionsize=R, aggregation=agg, persistent=persistent, block_threads=block_threads,
+                     xcd_slices=xcd)
         run = MultiRun(work, P, R)
         tun = _tuning(point)
         run.run(3, tun)  # warm up into the steady state
@@ -173,13 +175,14 @@ def run_default(log=None, steps: int = 100, tune_steps: int = 20, space=None) ->
             log(f"multi tune {point}")
 
     widest = max(space["aggregation"])
+    xcds = space.get("xcd_slices", (8,))
     for P, R in itertools.product(space["pagesize"], space["regionsize"]):
         for persistent, bt in space["launch"]:
-            measure(P, R, widest, persistent, bt)
+            measure(P, R, widest, persistent, bt, xcds[0])
     b = max(points, key=lambda p: p["ops_per_s"])
-    for agg in space["aggregation"]:
-        if agg != widest:
-            measure(b["pagesize"], b["regionsize"], agg, b["persistent"], b["block_threads"])
+    for agg, xcd in itertools.product(space["aggregation"], xcds):
+        if (agg, xcd) != (widest, xcds[0]):
+            measure(b["pagesize"], b["regionsize"], agg, b["persistent"], b["block_threads"], xcd)
     best = max(points, key=lambda p: p["ops_per_s"])
     tune_s = time.time() - t0
     run = MultiRun(work, best["pagesize"], best["regionsize"])
