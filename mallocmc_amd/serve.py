@@ -206,8 +206,10 @@ def run_default(log=None, duration_s: float = 30.0, check_sample: int = 64) -> d
     import torch
     # one access block over the whole heap: ScatterAlloc fills the current
     # access block before it moves on, and with ~2 GiB live a 2 GiB block
-    # turns every block claim into a scan of a nearly full block
-    cfg = mmc.ScatterConfig(pagesize=16384, accessblocksize=32 * GiB, regionsize=16, resetfreedpages=True,
+    # turns every block claim into a scan of a nearly full block.  Pages of
+    # 64 KiB hold 16 blocks each, so a wave's block claims touch fewer pages:
+    # p50 37 us against 45 us at 16 KiB pages and 81 us at 4 KiB
+    cfg = mmc.ScatterConfig(pagesize=65536, accessblocksize=32 * GiB, regionsize=32, resetfreedpages=True,
                             flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
     alloc = mmc.ScatterAllocator(cfg, 32 * GiB)
     slots0 = alloc.getAvailableSlots(4096)
@@ -233,7 +235,7 @@ def run_default(log=None, duration_s: float = 30.0, check_sample: int = 64) -> d
     total_s = float(ms.sum()) / 1e3
     out = {
         "workload": f"serving loop: Poisson 2000 req/s for {duration_s:.0f} s in 10 ms steps, lognormal(6.2,0.8) tokens "
-                    "<= 8192, 256 B/token, 4 KiB blocks, 32 GiB heap (one access block), pagesize 16 KiB",
+                    "<= 8192, 256 B/token, 4 KiB blocks, 32 GiB heap (one access block), pagesize 64 KiB, regionsize 32",
         "timing": "device clock stamp at every step start; 100 steps per graph replay (one host launch "
                   "per 100 steps); p50/p99 over the arrival steps",
         "requests": int(len(trace.arrival)), "steps": int(len(ms)), "arrival_steps": arrival_steps,
