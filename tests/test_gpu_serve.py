@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 MiB = 1024 ** 2
 
 
-def _alloc(mmc, heap=256 * MiB):
-    cfg = mmc.ScatterConfig(pagesize=16384, accessblocksize=64 * MiB, regionsize=16, resetfreedpages=True,
+def _alloc(mmc, heap=256 * MiB, pagesize=16384, regionsize=16):
+    cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=64 * MiB, regionsize=regionsize, resetfreedpages=True,
                             flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
     return mmc.ScatterAllocator(cfg, heap)
 
@@ -33,9 +33,10 @@ def _run_all(loop, trace, chunk=512, max_steps=200000):
     return loop.stats()
 
 
-def test_serve_checksums_and_restore(mmc, cuda):
+@pytest.mark.parametrize("pagesize,regionsize", [(16384, 16), (65536, 32)])
+def test_serve_checksums_and_restore(mmc, cuda, pagesize, regionsize):
     from mallocmc_amd import serve
-    a = _alloc(mmc)
+    a = _alloc(mmc, pagesize=pagesize, regionsize=regionsize)
     s0 = a.getAvailableSlots(4096)
     loop = serve.ServeLoop(a, max_slots=4096, max_tokens=2048)
     trace = serve.make_trace(seed=3, rate_per_s=2000, step_s=0.01, duration_s=1.0, mu=5.0, sigma=0.8,
