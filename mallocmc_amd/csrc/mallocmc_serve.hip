@@ -267,10 +267,27 @@ namespace
 
     constexpr u32 kCompleteBlocks = 1024u;
 
-    //! Returns a finished request's blocks (wave-cooperative free), slot and
-    //! reservation; the whole wave calls it.  The results that the admission
-    //! reads are returning atomics, waited for here.
-    __device__ void release(const ServeView& v, u32 slot, u32 len)
+    //! Returns a finished request's slot and reservation (lane 0 of the
+    //! calling wave).  The results that the admission reads are returning
+    //! atomics, waited for here.
+    __device__ void releaseSlot(const ServeView& v, u32 slot, u32 len)
+    {
+        if(hw::laneid() != 0u)
+            return;
+        const u32 nb = divup(len, v.tokensPerBlock);
+        atom::add(&v.sc->counters[cCompleted], u64{1});
+        const u64 r = atom::sub(&v.sc->reserved, static_cast<u64>(nb) * v.blockBytes);
+        const u32 q = atom::exch(&v.slotReq[slot], kEmpty);
+        const u32 pos = atom::add(&v.sc->freeTop, 1u);
+        const u32 e = atom::exch(&v.freeStack[pos], slot);
+        hw::consume(static_cast<u32>(r) ^ q ^ e);
+    }
+
+    //! Frees a finished request's blocks (wave-cooperative free); the whole
+    //! wave calls it.  Nothing in the completion launch allocates, so the
+    //! frees need not come before the admission: the next step's decode
+    //! (the next launch) is the first to reuse the blocks.
+    __device__ void freeBlocks(const ServeView& v, u32 slot, u32 len)
     {
         const u32 lane = hw::laneid();
         const WaveAlloc wa{v.heap, WaveTuning{}};
@@ -287,15 +304,7 @@ namespace
         for(int d = 32; d > 0; d >>= 1)
             freed += __shfl_xor(freed, d, 64);
         if(lane == 0u)
-        {
-            atom::add(&v.sc->counters[cCompleted], u64{1});
             atom::add(&v.sc->counters[cFrees], static_cast<u64>(freed));
-            const u64 r = atom::sub(&v.sc->reserved, static_cast<u64>(nb) * v.blockBytes);
-            const u32 q = atom::exch(&v.slotReq[slot], kEmpty);
-            const u32 pos = atom::add(&v.sc->freeTop, 1u);
-            const u32 e = atom::exch(&v.freeStack[pos], slot);
-            hw::consume(static_cast<u32>(r) ^ q ^ e);
-        }
     }
 
     //! Step part 2, completion: the paged buffers of the finished requests,
@@ -377,9 +386,11 @@ namespace
                 // the request's first chunk: its last wave releases the
                 // request right away, beside the checksum reads (nothing in
                 // this launch allocates, and a free leaves the payload as it
-                // is, so the reads stay valid); the wave that counts the
-                // step's last release admits the next step's arrivals
-                release(v, slot, len);
+                // is, so the reads stay valid): slot and reservation first,
+                // then, after the admission that needs only those, its
+                // blocks.  The wave that counts the step's last slot return
+                // admits the next step's arrivals.
+                releaseSlot(v, slot, len);
                 u32 last = 0u;
                 if(lane == 0u)
                     last = atom::add(&v.sc->released, 1u) + 1u == nDone ? 1u : 0u;
@@ -393,6 +404,7 @@ namespace
                     }
                     admitWave(v);
                 }
+                freeBlocks(v, slot, len);
             }
             const u32 t0 = (c - base) * kTokensPerChunk;
             const u32 t1 = umin32(len, t0 + kTokensPerChunk);
