@@ -114,8 +114,9 @@ typedef struct mmc_step_desc {
  * tiles from d_tile_cursor[0].  tuning->persistent = 0: one workgroup of
  * tuning->block_threads threads per block_threads/64 consecutive tiles; when
  * they lie in one heap the workgroup first reorders its frees by page and its
- * mallocs by size (unless tuning->no_partition).  Which form is faster depends on the heaps' fill
- * (the multi-heap driver autotunes it). */
+ * mallocs by size (unless tuning->no_partition).  The workgroup form
+ * measured faster at every fill tried; the multi-heap driver autotunes the
+ * form and the workgroup size. */
 int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tuning, void* stream);
 
 #ifdef __cplusplus
