@@ -13,11 +13,12 @@
  *      the heap when its last block is full (wave-cooperative malloc), and
  *      writes the token's payload words;
  *   3. completion: a request that reached its length reads its whole buffer
- *      back into a 64-bit checksum, frees its blocks (wave-cooperative free)
- *      and returns its slot and reservation.
- * A step is two launches: decode, then completion, where the wave that
- * makes the step's last release also runs the admission for the next step
- * (the first admission runs at load).  Both can be captured in one graph; the step counter lives in
+ *      back into a 64-bit checksum and frees its blocks (wave-cooperative
+ *      free); its slot and reservation came back in decode.
+ * A step is two launches: decode, which also returns the slots and
+ * reservations of the requests it finishes, then completion, whose first
+ * wave runs the admission for the next step (the first admission runs at
+ * load).  Both can be captured in one graph; the step counter lives in
  * device memory.  Payload word w of token t of request r is
  * mmc_serve_word(r, t, w) below, so every checksum has a closed form.
  */

@@ -31,11 +31,11 @@ namespace
         u64 reserved;
         u32 cursor;
         u32 freeTop;
-        u32 nDone;
-        u32 nChunks;  // checksum chunks of the requests finished this step (decode adds them)
+        u32 nDone[2];   // entries of the two done lists (a step uses list step & 1)
+        u32 nChunks[2]; // checksum chunks of their requests (decode adds them)
         u64 counters[cCount];
         u32 nStamps;  // device-clock stamps recorded since load
-        u32 released; // requests of this step's done list released so far
+        u32 parity;   // the done list this step's decode filled (completion reads it)
     };
 
     constexpr u32 kTokensPerChunk = 64u;
@@ -50,9 +50,10 @@ namespace
         u32* slotLen;
         u32* slotCur;
         u64* slotBlocks;
-        u32* doneList; // slots of the requests finished this step, in decode's order
+        u32* doneList; // slots of the requests finished in a step, in decode's order
         u32* doneLen;  // their lengths and request indices, so that completion
-        u32* doneReq;  // reads an entry in one round trip
+        u32* doneReq;  // reads an entry in one round trip; two lists of maxSlots
+                       // entries each, alternating by step
         const u32* arrival;
         const u32* length;
         const u32* ids;
@@ -80,14 +81,13 @@ namespace
 
     //! One wave: admit the requests that have arrived by step sc->step, in
     //! arrival order, while a slot and budget are available; then advance
-    //! the step.  Runs at the end of the previous step's completion kernel (or
-    //! at load for step 0), after every slot and reservation of that step came
-    //! back.  Up to 64 candidates per round: their projected bytes are summed
-    //! with a prefix scan, the leading run that fits the budget and the free
-    //! slots is admitted at once, so a round costs a few round trips however
-    //! many requests it admits.  Slot state is written with device-scope
-    //! atomics: the completion kernel's other blocks wrote it in the same
-    //! launch from other XCDs.
+    //! the step.  Runs at the start of the previous step's completion kernel
+    //! (or at load for step 0): that step's decode already returned the
+    //! slots and reservations of the requests it finished.  Up to 64
+    //! candidates per round: their projected bytes are summed with a prefix
+    //! scan, the leading run that fits the budget and the free slots is
+    //! admitted at once, so a round costs a few round trips however many
+    //! requests it admits.  Slot state is written with device-scope atomics.
     __device__ void admitWave(const ServeView& v)
     {
         ServeScalars* sc = v.sc;
@@ -184,9 +184,13 @@ namespace
             stamp(v.sc, v.stamps, v.stampCapacity);
         const u32 slot = blockIdx.x * blockDim.x + threadIdx.x;
         const bool valid = slot < v.maxSlots;
-        // the slot's state in one round trip (a free slot's position and
-        // length are not used), then the request id and the current block
-        // in the next, issued before the malloc's own loads
+        // the step (its parity picks the done list), read beside the slot's
+        // state in one round trip (a free slot's position and length are not
+        // used), then the request id and the current block in the next,
+        // issued before the malloc's own loads
+        u64 stepNow = 0u;
+        if(hw::laneid() == 0u)
+            stepNow = atom::load(&v.sc->step);
         u32 req = kEmpty, t = 0u, len = 0u;
         if(valid)
         {
@@ -205,15 +209,33 @@ namespace
             if(!needBlock)
                 off = blocks[t / v.tokensPerBlock];
         }
+        const u32 par = static_cast<u32>(hw::readlane64(stepNow, 0u)) & 1u;
+        if(blockIdx.x == 0u && threadIdx.x == 0u)
+        {
+            // the other list was read by the previous step's completion and
+            // is filled by the next step's decode
+            atom::store(&v.sc->nDone[par ^ 1u], 0u);
+            atom::store(&v.sc->nChunks[par ^ 1u], 0u);
+            atom::store(&v.sc->parity, par);
+        }
         // a request that reaches its length this step takes its done-list
-        // entry now: the returning atomic travels with the malloc's loads
-        // instead of adding a round trip after them
+        // entry and returns its slot and reservation now: the returning
+        // atomics travel with the malloc's loads instead of adding round
+        // trips after them.  The slot may be handed out again by this step's
+        // admission (in completion), which writes the slot's request,
+        // length and position but not its block list; completion reads the
+        // finished request from the done list and its blocks from the block
+        // list, and frees the blocks.
         const bool finishing = active && t + 1u == len;
-        u32 doneAt = 0u;
+        u32 doneAt = 0u, freeAt = 0u;
+        const u32 list = par * v.maxSlots;
         if(finishing)
         {
-            doneAt = atom::add(&v.sc->nDone, 1u);
-            atom::add(&v.sc->nChunks, divup(len, kTokensPerChunk));
+            doneAt = atom::add(&v.sc->nDone[par], 1u);
+            atom::add(&v.sc->nChunks[par], divup(len, kTokensPerChunk));
+            freeAt = atom::add(&v.sc->freeTop, 1u);
+            atom::sub(&v.sc->reserved, static_cast<u64>(divup(len, v.tokensPerBlock)) * v.blockBytes);
+            atom::add(&v.sc->counters[cCompleted], u64{1});
         }
         const WaveAlloc wa{v.heap, WaveTuning{}};
         void* p = wa.malloc<NoopDistribution>(needBlock ? v.blockBytes : 0u);
@@ -245,9 +267,11 @@ namespace
             v.slotCur[slot] = t + 1u;
             if(finishing)
             {
-                v.doneList[doneAt] = slot;
-                v.doneLen[doneAt] = len;
-                v.doneReq[doneAt] = req;
+                v.doneList[list + doneAt] = slot;
+                v.doneLen[list + doneAt] = len;
+                v.doneReq[list + doneAt] = req;
+                atom::store(&v.slotReq[slot], kEmpty);
+                atom::store(&v.freeStack[freeAt], slot);
             }
         }
         u32 mallocs = needBlock ? 1u : 0u;
@@ -266,22 +290,6 @@ namespace
     }
 
     constexpr u32 kCompleteBlocks = 1024u;
-
-    //! Returns a finished request's slot and reservation (lane 0 of the
-    //! calling wave).  The results that the admission reads are returning
-    //! atomics, waited for here.
-    __device__ void releaseSlot(const ServeView& v, u32 slot, u32 len)
-    {
-        if(hw::laneid() != 0u)
-            return;
-        const u32 nb = divup(len, v.tokensPerBlock);
-        atom::add(&v.sc->counters[cCompleted], u64{1});
-        const u64 r = atom::sub(&v.sc->reserved, static_cast<u64>(nb) * v.blockBytes);
-        const u32 q = atom::exch(&v.slotReq[slot], kEmpty);
-        const u32 pos = atom::add(&v.sc->freeTop, 1u);
-        const u32 e = atom::exch(&v.freeStack[pos], slot);
-        hw::consume(static_cast<u32>(r) ^ q ^ e);
-    }
 
     //! Frees a finished request's blocks (wave-cooperative free); the whole
     //! wave calls it.  Nothing in the completion launch allocates, so the
@@ -325,19 +333,20 @@ namespace
         __shared__ u64 partial[4];
         const u32 lane = hw::laneid();
         const u32 q = threadIdx.x / 64u;
-        const u32 nDone = atom::load(&v.sc->nDone);
-        const u32 total = atom::load(&v.sc->nChunks);
+        // decode returned every finished request's slot and reservation, so
+        // the first wave admits the next step's arrivals at once
+        if(blockIdx.x == 0u && q == 0u)
+            admitWave(v);
+        // the done list decode filled (both lists' counts in the same round
+        // trip as the parity; admission does not touch them)
+        const u32 par = atom::load(&v.sc->parity);
+        const u32 n0 = atom::load(&v.sc->nDone[0]), n1 = atom::load(&v.sc->nDone[1]);
+        const u32 c0 = atom::load(&v.sc->nChunks[0]), c1 = atom::load(&v.sc->nChunks[1]);
+        const u32 nDone = par != 0u ? n1 : n0;
+        const u32 total = par != 0u ? c1 : c0;
+        const u32 list = par * v.maxSlots;
         if(blockIdx.x >= total)
-        {
-            // no chunk here; with none at all, the first workgroup admits
-            if(total == 0u && blockIdx.x == 0u && q == 0u)
-            {
-                if(lane == 0u)
-                    atom::store(&v.sc->nDone, 0u);
-                admitWave(v);
-            }
-            return;
-        }
+            return; // no chunk here
         const u32 wpt = v.tokenBytes / 4u;
         u32 i = 0u, base = 0u; // done-list entry and its first chunk id
         for(u32 c = blockIdx.x; c < total; c += gridDim.x)
@@ -351,9 +360,9 @@ namespace
             while(i < nDone)
             {
                 const u32 e = i + lane;
-                const u32 es = e < nDone ? v.doneList[e] : 0u;
-                const u32 el = e < nDone ? v.doneLen[e] : 0u;
-                const u32 er = e < nDone ? v.doneReq[e] : 0u;
+                const u32 es = e < nDone ? v.doneList[list + e] : 0u;
+                const u32 el = e < nDone ? v.doneLen[list + e] : 0u;
+                const u32 er = e < nDone ? v.doneReq[list + e] : 0u;
                 const u32 n = divup(el, kTokensPerChunk);
                 u32 incl = n;
                 for(u32 d = 1u; d < 64u; d <<= 1)
@@ -383,27 +392,10 @@ namespace
             const u64* blocks = v.slotBlocks + static_cast<u64>(slot) * v.maxBlocks;
             if(c == base && q == 3u)
             {
-                // the request's first chunk: its last wave releases the
-                // request right away, beside the checksum reads (nothing in
-                // this launch allocates, and a free leaves the payload as it
-                // is, so the reads stay valid): slot and reservation first,
-                // then, after the admission that needs only those, its
-                // blocks.  The wave that counts the step's last slot return
-                // admits the next step's arrivals.
-                releaseSlot(v, slot, len);
-                u32 last = 0u;
-                if(lane == 0u)
-                    last = atom::add(&v.sc->released, 1u) + 1u == nDone ? 1u : 0u;
-                if(__builtin_amdgcn_readfirstlane(last) != 0u)
-                {
-                    if(lane == 0u)
-                    {
-                        atom::store(&v.sc->released, 0u);
-                        atom::store(&v.sc->nChunks, 0u);
-                        atom::store(&v.sc->nDone, 0u);
-                    }
-                    admitWave(v);
-                }
+                // the request's first chunk: its last wave frees the
+                // request's blocks right away, beside the checksum reads
+                // (nothing in this launch allocates, and a free leaves the
+                // payload as it is, so the reads stay valid)
                 freeBlocks(v, slot, len);
             }
             const u32 t0 = (c - base) * kTokensPerChunk;
@@ -461,9 +453,11 @@ namespace
             v.sc->reserved = 0u;
             v.sc->cursor = 0u;
             v.sc->freeTop = v.maxSlots;
-            v.sc->nDone = 0u;
-            v.sc->nChunks = 0u;
-            v.sc->released = 0u;
+            v.sc->nDone[0] = 0u;
+            v.sc->nDone[1] = 0u;
+            v.sc->nChunks[0] = 0u;
+            v.sc->nChunks[1] = 0u;
+            v.sc->parity = 0u;
             v.sc->nStamps = 0u;
             for(u32 c = 0; c < cCount; ++c)
                 v.sc->counters[c] = 0u;
@@ -510,7 +504,7 @@ int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** ou
     s->cfg = *cfg;
     const u32 maxBlocks = divup(cfg->max_tokens, cfg->tokens_per_block);
     const u64 n = cfg->max_slots;
-    const u64 bytes = sizeof(ServeScalars) + 4u * n * 7u + 8u * n * maxBlocks + 256u;
+    const u64 bytes = sizeof(ServeScalars) + 4u * n * 10u + 8u * n * maxBlocks + 256u;
     hipError_t err = hipMalloc(&s->arena, bytes);
     if(err != hipSuccess)
     {
@@ -533,9 +527,9 @@ int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** ou
     v.slotCur = reinterpret_cast<u32*>(a);
     a += 4u * n;
     v.doneList = reinterpret_cast<u32*>(a);
-    a += 4u * n;
+    a += 8u * n;
     v.doneLen = reinterpret_cast<u32*>(a);
-    a += 4u * n;
+    a += 8u * n;
     v.doneReq = reinterpret_cast<u32*>(a);
     v.maxSlots = cfg->max_slots;
     v.blockBytes = cfg->block_bytes;
