@@ -533,6 +533,73 @@ namespace
         }
     }
 
+    //! The fill with each workgroup's requests reordered by size before
+    //! every round (partitionByKey): lane i's request may be served by
+    //! another lane, which reports a null back to lane i through LDS.  The
+    //! rounds are workgroup-uniform (the partition has barriers): a
+    //! workgroup runs while any of its lanes is active and no stop was seen.
+    //! Each lane draws the same sizes as in fillKernel.
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) void fillPartitionKernel(
+        ScatterHeap h,
+        WaveTuning t,
+        mmc_fill_desc d)
+    {
+        __shared__ u32 sStopped[256];
+        __shared__ u32 sStop;
+        const u32 lane = blockIdx.x * blockDim.x + threadIdx.x;
+        bool active = lane < d.lanes;
+        const WaveAlloc wa{h, t};
+        for(u32 it = 0;; ++it)
+        {
+            if(threadIdx.x == 0u)
+                sStop = d.d_stop ? atom::load(d.d_stop) : 0u;
+            sStopped[threadIdx.x] = 0u;
+            if(__syncthreads_or(active ? 1 : 0) == 0 || sStop != 0u)
+                break;
+            u32 bytes = 0u;
+            if(active)
+            {
+                const u64 r = mix64(d.seed ^ mix64((static_cast<u64>(lane) << 24) + it));
+                const u32 coin = static_cast<u32>(r % 1000u);
+                const u32 u = static_cast<u32>(r >> 32);
+                if(coin < d.p_small_permille)
+                    bytes = d.small_lo + u % (d.small_hi - d.small_lo + 1u);
+                else
+                    bytes = d.large_lo + u % (d.large_hi - d.large_lo + 1u);
+            }
+            u32 slot = threadIdx.x;
+            partitionByKey<u32, 256>(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
+            void* p = wa.malloc<NoopDistribution>(bytes);
+            if(bytes != 0u)
+            {
+                if(p == nullptr)
+                {
+                    sStopped[slot] = 1u;
+                    if(d.d_stop)
+                        atom::store(d.d_stop, 1u);
+                }
+                else
+                {
+                    const u64 pos = atom::add(d.d_count, u64{1});
+                    if(pos < d.cap)
+                    {
+                        d.d_offsets[pos] = h.offsetOf(p);
+                        d.d_sizes[pos] = bytes;
+                    }
+                    else
+                    {
+                        h.destroy<WaveIds>(p);
+                        sStopped[slot] = 1u;
+                    }
+                }
+            }
+            __syncthreads();
+            if(sStopped[threadIdx.x] != 0u)
+                active = false;
+            __syncthreads();
+        }
+    }
+
     template<u32 B>
     void launchMultiBlock(bool xm, const ScatterHeap* heaps, const mmc_step_desc& d, const WaveTuning& t,
                           bool part, hipStream_t s)
@@ -670,8 +737,12 @@ int mmc_wave_fill_until_null(mmc_heap* heap, const mmc_fill_desc* desc, const mm
        || desc->small_hi < desc->small_lo || desc->large_hi < desc->large_lo || desc->p_small_permille > 1000u)
         return mmc::fail(MMC_EINVAL, "bad fill descriptor");
     const WaveTuning t = toTuning(tuning);
-    hipLaunchKernelGGL(fillKernel, dim3(mmc::gridFor(desc->lanes, 256)), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), heap->view, t, *desc);
+    if(partitionOf(tuning))
+        hipLaunchKernelGGL(fillPartitionKernel, dim3(mmc::gridFor(desc->lanes, 256)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), heap->view, t, *desc);
+    else
+        hipLaunchKernelGGL(fillKernel, dim3(mmc::gridFor(desc->lanes, 256)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), heap->view, t, *desc);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "fill launch");
 }
