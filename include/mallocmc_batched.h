@@ -31,9 +31,9 @@ typedef struct mmc_tuning {
     uint32_t no_partition; /* 1: lane i of a non-persistent launch serves request i (0, the default:
                               each block first reorders its requests by size, its frees by page,
                               so that a wave meets few size classes / pages) */
-    uint32_t block_threads; /* threads per block of the non-persistent wave malloc and free and of
-                               the non-persistent multi-heap step, the requests one partition
-                               sorts: 256, 512 or 1024 (0 = 1024) */
+    uint32_t block_threads; /* threads per block of the non-persistent wave malloc and free, of the
+                               non-persistent multi-heap step and of the fill, the requests one
+                               partition sorts: 256, 512 or 1024 (0 = 1024) */
 } mmc_tuning;
 
 int mmc_default_tuning(mmc_tuning* t);

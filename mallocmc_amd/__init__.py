@@ -70,7 +70,7 @@ class Tuning:
     the region scan walks every window (no_skip) and whether a block serves
     its requests in arrival order instead of partitioned by size
     (no_partition), and the threads per block of the non-persistent wave
-    malloc and free and of the non-persistent multi-heap step
+    malloc and free, of the non-persistent multi-heap step and of the fill
     (block_threads: 256, 512 or 1024; 0 = 1024)."""
     aggregation: int = 64
     xcd_slices: int = 0
