@@ -17,8 +17,8 @@
  *      free); its slot and reservation came back in decode.
  * A step is two launches: decode, which also returns the slots and
  * reservations of the requests it finishes, then completion, whose first
- * wave runs the admission for the next step (the first admission runs at
- * load).  Both can be captured in one graph; the step counter lives in
+ * workgroup runs the admission for the next step (the first admission runs
+ * at load).  Both can be captured in one graph; the step counter lives in
  * device memory.  Payload word w of token t of request r is
  * mmc_serve_word(r, t, w) below, so every checksum has a closed form.
  */

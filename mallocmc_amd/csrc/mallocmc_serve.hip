@@ -289,10 +289,10 @@ namespace
         }
     }
 
-    //! Completion workgroups per step (a step finishes a few hundred 64-token
-    //! chunks; one workgroup takes every 512th chunk): 512 measured p50 31.1
-    //! us, p99 45.8 us, against 32.3 / 48.0 us with 1024 and 31.0 / 46.4 us
-    //! with 256.
+    //! Checksum workgroups per step, beside the admission workgroup (a step
+    //! finishes a few hundred 64-token chunks; one workgroup takes every
+    //! 512th chunk): 512 measured p50 31.1 us, p99 45.8 us, against 32.3 /
+    //! 48.0 us with 1024 and 31.0 / 46.4 us with 256.
     constexpr u32 kCompleteBlocks = 512u;
 
     //! Frees a finished request's blocks (wave-cooperative free); the whole
@@ -324,8 +324,9 @@ namespace
     //! their checksums (lane = word of a token, the 4 waves of a workgroup take
     //! every fourth token, all 16 of a wave's tokens in flight at once;
     //! partial sums add atomically).  Decode already returned the finished
-    //! requests' slots and reservations, so the first wave admits the next
-    //! step's arrivals at once, and one step is two launches.  The workgroup
+    //! requests' slots and reservations, so workgroup 0 admits the next
+    //! step's arrivals at once (and reads no chunk), and one step is two
+    //! launches.  The workgroup
     //! holding a request's first chunk frees the request's blocks beside the
     //! reads.  Workgroups without a chunk (decode added up the step's chunk
     //! count) leave at once and touch no shared word: a ticket taken by every
@@ -337,8 +338,14 @@ namespace
         const u32 q = threadIdx.x / 64u;
         // decode returned every finished request's slot and reservation, so
         // the first wave admits the next step's arrivals at once
-        if(blockIdx.x == 0u && q == 0u)
-            admitWave(v);
+        // workgroup 0 admits and does nothing else, so no chunk waits for it
+        if(blockIdx.x == 0u)
+        {
+            if(q == 0u)
+                admitWave(v);
+            return;
+        }
+        const u32 first = blockIdx.x - 1u, nWork = gridDim.x - 1u;
         // the done list decode filled (both lists' counts in the same round
         // trip as the parity; admission does not touch them)
         const u32 par = atom::load(&v.sc->parity);
@@ -347,11 +354,11 @@ namespace
         const u32 nDone = par != 0u ? n1 : n0;
         const u32 total = par != 0u ? c1 : c0;
         const u32 list = par * v.maxSlots;
-        if(blockIdx.x >= total)
+        if(first >= total)
             return; // no chunk here
         const u32 wpt = v.tokenBytes / 4u;
         u32 i = 0u, base = 0u; // done-list entry and its first chunk id
-        for(u32 c = blockIdx.x; c < total; c += gridDim.x)
+        for(u32 c = first; c < total; c += nWork)
         {
             // the entry holding chunk c: 64 entries per round trip (lane k
             // reads entry i + k: slot, length and request side by side), a
@@ -578,7 +585,7 @@ int mmc_serve_step(mmc_serve* s, void* stream)
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(decodeKernel, dim3(mmc::gridFor(s->cfg.max_slots, kDecodeThreads)), dim3(kDecodeThreads), 0, st,
                        s->view);
-    hipLaunchKernelGGL(completeKernel, dim3(kCompleteBlocks), dim3(256), 0, st, s->view);
+    hipLaunchKernelGGL(completeKernel, dim3(kCompleteBlocks + 1u), dim3(256), 0, st, s->view);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve step launch");
 }
