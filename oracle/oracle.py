@@ -71,41 +71,53 @@ def build() -> str:
     return _LIB_PATH
 
 
-_lib = None
+_libs = {}
+
+
+def _load(path: str):
+    L = _libs.get(path)
+    if L is not None:
+        return L
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    L.orc_default_config.argtypes = [P(OrcConfig)]
+    L.orc_create.argtypes = [P(OrcConfig), ctypes.c_uint64, P(ctypes.c_char_p)]
+    L.orc_create.restype = ctypes.c_void_p
+    L.orc_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_get_geometry.argtypes = [ctypes.c_void_p, P(OrcGeometry)]
+    L.orc_compute_geometry.argtypes = [P(OrcConfig), ctypes.c_uint64, P(OrcGeometry)]
+    L.orc_compute_geometry.restype = ctypes.c_int
+    L.orc_set_atomic_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc_malloc.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    L.orc_malloc.restype = ctypes.c_uint64
+    L.orc_free.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    L.orc_available_slots.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    L.orc_available_slots.restype = ctypes.c_uint32
+    L.orc_run_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    L.orc_read_pte.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [P(ctypes.c_uint32)] * 3
+    L.orc_read_region.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.orc_read_region.restype = ctypes.c_uint32
+    L.orc_read_scalars.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), P(ctypes.c_uint32)]
+    L.orc_pool.argtypes = [ctypes.c_void_p]
+    L.orc_pool.restype = ctypes.c_void_p
+    L.orc_bench_concurrent.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, P(OrcBenchResult)]
+    L.orc_bench_concurrent.restype = ctypes.c_int
+    _libs[path] = L
+    return L
 
 
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            build()
-        L = ctypes.CDLL(_LIB_PATH)
-        P = ctypes.POINTER
-        L.orc_default_config.argtypes = [P(OrcConfig)]
-        L.orc_create.argtypes = [P(OrcConfig), ctypes.c_uint64, P(ctypes.c_char_p)]
-        L.orc_create.restype = ctypes.c_void_p
-        L.orc_destroy.argtypes = [ctypes.c_void_p]
-        L.orc_get_geometry.argtypes = [ctypes.c_void_p, P(OrcGeometry)]
-        L.orc_compute_geometry.argtypes = [P(OrcConfig), ctypes.c_uint64, P(OrcGeometry)]
-        L.orc_compute_geometry.restype = ctypes.c_int
-        L.orc_set_atomic_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.orc_malloc.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
-        L.orc_malloc.restype = ctypes.c_uint64
-        L.orc_free.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
-        L.orc_available_slots.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
-        L.orc_available_slots.restype = ctypes.c_uint32
-        L.orc_run_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
-        L.orc_read_pte.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [P(ctypes.c_uint32)] * 3
-        L.orc_read_region.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
-        L.orc_read_region.restype = ctypes.c_uint32
-        L.orc_read_scalars.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), P(ctypes.c_uint32)]
-        L.orc_pool.argtypes = [ctypes.c_void_p]
-        L.orc_pool.restype = ctypes.c_void_p
-        L.orc_bench_concurrent.argtypes = [
-            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, P(OrcBenchResult)]
-        L.orc_bench_concurrent.restype = ctypes.c_int
-        _lib = L
-    return _lib
+    """The faithful oracle build."""
+    return _load(_LIB_PATH)
+
+
+def mutant_lib(n: int):
+    """Test-only mutant build n (scatter_oracle.c, ORC_MUTANT): one stateful
+    heuristic broken, for showing that the known-answer tests catch it."""
+    return _load(os.path.join(_HERE, "_build", f"libscatter_oracle_mut{n}.so"))
 
 
 def make_config(pagesize=4096, regionsize=16, accessblocksize=2 * 1024 ** 3, wastefactor=2,
@@ -127,17 +139,18 @@ class OracleHeap:
     """One CPU heap; methods mirror DeviceAllocator::malloc/free and
     Allocator::getAvailableSlots on the reference's CPU backend."""
 
-    def __init__(self, cfg: OrcConfig, memsize: int):
+    def __init__(self, cfg: OrcConfig, memsize: int, library=None):
         err = ctypes.c_char_p()
         self._cfg = cfg
-        self._h = lib().orc_create(ctypes.byref(cfg), memsize, ctypes.byref(err))
+        self._L = library if library is not None else lib()
+        self._h = self._L.orc_create(ctypes.byref(cfg), memsize, ctypes.byref(err))
         if not self._h:
             raise ValueError((err.value or b"orc_create failed").decode())
         self.memsize = memsize
 
     def close(self):
         if self._h:
-            lib().orc_destroy(self._h)
+            self._L.orc_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -152,49 +165,49 @@ class OracleHeap:
     @property
     def geometry(self) -> OrcGeometry:
         g = OrcGeometry()
-        lib().orc_get_geometry(self._h, ctypes.byref(g))
+        self._L.orc_get_geometry(self._h, ctypes.byref(g))
         return g
 
     def set_atomic_mode(self, mode: int):
-        lib().orc_set_atomic_mode(self._h, mode)
+        self._L.orc_set_atomic_mode(self._h, mode)
 
     def malloc(self, nbytes: int):
-        off = lib().orc_malloc(self._h, nbytes)
+        off = self._L.orc_malloc(self._h, nbytes)
         return None if off == ORC_NULL else off
 
     def free(self, offset):
-        lib().orc_free(self._h, ORC_NULL if offset is None else offset)
+        self._L.orc_free(self._h, ORC_NULL if offset is None else offset)
 
     def available_slots(self, size: int) -> int:
-        return lib().orc_available_slots(self._h, size)
+        return self._L.orc_available_slots(self._h, size)
 
     def run_stream(self, ops) -> np.ndarray:
         ops = np.ascontiguousarray(ops, dtype=np.int64)
         out = np.zeros(len(ops), dtype=np.uint64)
-        lib().orc_run_stream(self._h, ops.ctypes.data, len(ops), out.ctypes.data)
+        self._L.orc_run_stream(self._h, ops.ctypes.data, len(ops), out.ctypes.data)
         return out
 
     def pte(self, page: int):
         a, b, c = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
-        lib().orc_read_pte(self._h, page, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        self._L.orc_read_pte(self._h, page, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return a.value, b.value, c.value
 
     def region(self, r: int) -> int:
-        return lib().orc_read_region(self._h, r)
+        return self._L.orc_read_region(self._h, r)
 
     def scalars(self):
         a, b = ctypes.c_uint32(), ctypes.c_uint32()
-        lib().orc_read_scalars(self._h, ctypes.byref(a), ctypes.byref(b))
+        self._L.orc_read_scalars(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
     def pool_bytes(self, offset: int, n: int) -> bytes:
-        base = lib().orc_pool(self._h)
+        base = self._L.orc_pool(self._h)
         return ctypes.string_at(base + offset, n)
 
     def bench_concurrent(self, sizes, nthreads: int = 0):
         sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
         offs = np.zeros(len(sizes), dtype=np.uint64)
         res = OrcBenchResult()
-        lib().orc_bench_concurrent(self._h, sizes.ctypes.data, len(sizes), nthreads, offs.ctypes.data,
+        self._L.orc_bench_concurrent(self._h, sizes.ctypes.data, len(sizes), nthreads, offs.ctypes.data,
                                    ctypes.byref(res))
         return res, offs

@@ -21,6 +21,28 @@
 typedef uint32_t u32;
 typedef uint64_t u64;
 
+/* Test-only mutants.  The oracle's pinning tests (tests/test_oracle_kats.py)
+ * build this file once more per ORC_MUTANT value; each value breaks one
+ * stateful heuristic of the walk, the way a plausible misreading of
+ * Scatter.hpp would, so the tests can show that their known-answer streams
+ * catch it.  0, the default build, is the faithful restatement, and nothing
+ * but those tests loads a mutant build. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define ORC_MUT_NO_REGION_INCREMENT 1  /* drop the fill-level CAS, :612-617 */
+#define ORC_MUT_INCREMENT_BELOW_R 2    /* cap it at fill + 1 < R instead of <= R */
+#define ORC_MUT_NO_REGION_RESET 3      /* drop the exchange to 0, :727-729 */
+#define ORC_MUT_RESET_WHEN_EMPTY 4     /* reset when the page empties, not at P/2/c */
+#define ORC_MUT_NO_BLOCK_ADVANCE 5     /* drop the cursor advance, :620-629 */
+#define ORC_MUT_ADVANCE_ALWAYS 6       /* advance without the `>` guard */
+#define ORC_MUT_NO_BLOCK_MIN 7         /* drop the atomicMin of the cursor, :733-734 */
+#define ORC_MUT_SINGLE_PASS 8          /* no second pass, :634 */
+#define ORC_MUT_CHECKLEVEL_R 9         /* first pass gated at R instead of 3R/4, :553 */
+#define ORC_MUT_NO_FFPB_CAS 10         /* drop the _firstFreePageBased CAS, :796-801 */
+#define ORC_MUT_NO_FFPB_MAX 11         /* drop its atomicMax on free, :885 */
+#define MUT(name) (ORC_MUTANT == ORC_MUT_##name)
+
 struct orc_heap {
     orc_config cfg;
     /* constants, Scatter.hpp:173-187 */
@@ -332,8 +354,8 @@ static uint8_t* alloc_chunked(orc_heap* h, u32 bytes)
     const u32 maxchunksize = umin(P, umax(h->cfg.wastefactor * bytes, paddedMin));
     const u32 global_start = start_in_block + vload(&h->firstfreeblock) * pagesperblock;
 
-    u32 checklevel = R * 3u / 4u;
-    for(u32 finder = 0; finder < 2u; ++finder)
+    u32 checklevel = MUT(CHECKLEVEL_R) ? R : R * 3u / 4u;
+    for(u32 finder = 0; finder < (MUT(SINGLE_PASS) ? 1u : 2u); ++finder)
     {
         u32 global_page = global_start;
         do
@@ -360,14 +382,14 @@ static uint8_t* alloc_chunked(orc_heap* h, u32 bytes)
                     }
                     pir = region_offset + ((pir + 1u) % R);
                 } while(pir != global_page);
-                if(fill + 1u <= R)
+                if(!MUT(NO_REGION_INCREMENT) && fill + 1u <= R - (MUT(INCREMENT_BELOW_R) ? 1u : 0u))
                     cas(h, &h->regions[region], fill, fill + 1u);
             }
             global_page = (global_page + R) % numpages;
-            if(global_page % pagesperblock == 0u)
+            if(!MUT(NO_BLOCK_ADVANCE) && global_page % pagesperblock == 0u)
             {
                 const u32 abid = global_page / pagesperblock;
-                if(abid > vload(&h->firstfreeblock))
+                if(MUT(ADVANCE_ALWAYS) || abid > vload(&h->firstfreeblock))
                     vstore(&h->firstfreeblock, abid);
             }
         } while(global_page != global_start);
@@ -416,13 +438,15 @@ static void dealloc_chunked(orc_heap* h, uint8_t* mem, u32 page, u32 chunksize)
     else
         oldfill = rmw(h, PTE_COUNT(h, page), OP_SUB, 1u);
 
-    if(oldfill == P / 2u / chunksize)
+    if(oldfill == (MUT(RESET_WHEN_EMPTY) ? 1u : P / 2u / chunksize))
     {
         const u32 region = page / h->cfg.regionsize;
-        rmw(h, &h->regions[region], OP_EXCH, 0u);
+        if(!MUT(NO_REGION_RESET))
+            rmw(h, &h->regions[region], OP_EXCH, 0u);
         const u32 pagesperblock = h->numpages / h->accessblocks;
         const u32 block = page / pagesperblock;
-        rmw(h, &h->firstfreeblock, OP_MIN, block);
+        if(!MUT(NO_BLOCK_MIN))
+            rmw(h, &h->firstfreeblock, OP_MIN, block);
     }
 }
 
@@ -462,7 +486,7 @@ static uint8_t* alloc_page_based_single_region(orc_heap* h, u32 startpage, u32 e
             {
                 if(markpages(h, sp, pagestoalloc, bytes))
                 {
-                    if(!left_free)
+                    if(!left_free && !MUT(NO_FFPB_CAS))
                         cas(h, &h->firstFreePageBased, startpage, sp - 1u);
                     return page_data(h, sp);
                 }
@@ -500,7 +524,8 @@ static void dealloc_page_based(orc_heap* h, u32 page, u32 bytes)
     fence_device();
     for(u32 p = page; p < page + pages; ++p)
         cas(h, PTE_CHUNKSIZE(h, p), bytes, 0u);
-    rmw(h, &h->firstFreePageBased, OP_MAX, page + pages - 1u);
+    if(!MUT(NO_FFPB_MAX))
+        rmw(h, &h->firstFreePageBased, OP_MAX, page + pages - 1u);
 }
 
 /* create (Scatter.hpp:895-912); allocPageBased (:848-866) reduces to one call

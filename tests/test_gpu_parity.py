@@ -48,11 +48,11 @@ def test_serial_16B_offsets(mmc, oracle_mod, cuda):
     assert gs == ws
 
 
-@pytest.mark.parametrize("seed,pagesize,reset,region", [
-    (1, 4096, False, 16), (2, 4096, True, 16), (3, 16384, True, 8), (4, 16384, False, 4),
-    (5, 65536, True, 16), (6, 1024, False, 2),
+@pytest.mark.parametrize("seed,pagesize,reset,region,heap", [
+    (1, 4096, False, 16, 2 * MiB), (2, 4096, True, 16, 2 * MiB), (3, 16384, True, 8, 4 * MiB),
+    (4, 16384, False, 4, 2 * MiB), (5, 65536, True, 16, 32 * MiB), (6, 1024, False, 2, 1 * MiB),
 ])
-def test_serial_random_streams(mmc, oracle_mod, cuda, seed, pagesize, reset, region):
+def test_serial_random_streams(mmc, oracle_mod, cuda, seed, pagesize, reset, region, heap):
     """Mixed sizes (chunked, == pagesize, multi-page), interleaved frees, small heaps
     so that regions fill, the 3/4 threshold and OOM are reached."""
     rng = np.random.default_rng(seed)
@@ -60,13 +60,14 @@ def test_serial_random_streams(mmc, oracle_mod, cuda, seed, pagesize, reset, reg
                      pagesize + 16, 2 * pagesize, 3 * pagesize + 5])
     sizes = rng.choice(cand, size=4000)
     ops = wl.random_stream(rng, 3000, sizes, p_free=0.3)
-    heap = max(8 * pagesize * region * 4, 2 * MiB)
     cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=heap // 2, regionsize=region,
                             resetfreedpages=reset)
     want, got, ws, gs = _serial_pair(mmc, oracle_mod, cuda, cfg, heap, ops)
     np.testing.assert_array_equal(got, want)
     assert gs == ws
-    assert (want == -1).any() or True  # streams may or may not reach OOM
+    # every stream is sized to reach OOM, for multi-page requests and chunked ones
+    assert ((want == -1) & (ops > pagesize)).any()
+    assert ((want == -1) & (ops >= 0) & (ops <= pagesize)).any()
 
 
 @pytest.mark.parametrize("seed,pagesize,wastefactor,align,region", [

@@ -171,6 +171,14 @@ def _check_state(h, P, live, nb, WF=2, mc=16):
         assert count >= len(items)
 
 
+def _longest_free_run(free):
+    best = cur = 0
+    for f in free:
+        cur = cur + 1 if f else 0
+        best = max(best, cur)
+    return best
+
+
 @pytest.mark.parametrize("seed,P,R,reset", [(11, 4096, 4, True), (12, 4096, 2, False), (13, 16384, 4, True),
                                             (14, 1024, 2, True)])
 def test_bruteforce_invariants(oracle_mod, seed, P, R, reset):
@@ -183,7 +191,7 @@ def test_bruteforce_invariants(oracle_mod, seed, P, R, reset):
         g = h.geometry
         live = {}
         handles = []
-        nulls = 0
+        nulls = wrapped = 0
         for step in range(1500):
             if handles and rng.random() < 0.4:
                 j = int(rng.integers(len(handles)))
@@ -193,7 +201,8 @@ def test_bruteforce_invariants(oracle_mod, seed, P, R, reset):
             else:
                 b = int(rng.choice(cand))
                 pb = pad(b)
-                before_free_pages = sum(1 for p in range(g.numpages) if h.pte(p)[0] == 0)
+                free_before = [h.pte(p)[0] == 0 for p in range(g.numpages)]
+                ffpb_before = h.scalars()[0]
                 off = h.malloc(b)
                 if off is None:
                     nulls += 1
@@ -204,8 +213,15 @@ def test_bruteforce_invariants(oracle_mod, seed, P, R, reset):
                             assert cs != 0
                             if pb <= cs <= min(P, max(2 * pb, mc)) and cs <= P:
                                 assert cnt >= capacity(P, cs)
+                    elif ffpb_before < g.numpages:
+                        # null => no run of ceil(pb/P) free pages anywhere
+                        assert _longest_free_run(free_before) < math.ceil(pb / P)
                     else:
-                        assert before_free_pages < math.ceil(pb / P) or True
+                        # a run claimed at page 0 wrapped the cursor to 2^32-1,
+                        # after which the page-based scans look at no page
+                        # (Scatter.hpp:785, 796-801, 831-835)
+                        wrapped += 1
+                        assert ffpb_before == 2 ** 32 - 1
                     continue
                 assert off % 16 == 0 and off + pb <= g.numpages * P
                 for o, s in live.items():
@@ -219,7 +235,15 @@ def test_bruteforce_invariants(oracle_mod, seed, P, R, reset):
                 live[off] = pb
                 handles.append(off)
             _check_state(h, P, live, g.numpages, mc=mc)
+            # Every page above _firstFreePageBased belongs to a live multi-page
+            # run: the cursor only drops across the run a scan just claimed
+            # (:796-801) and a released run raises it to its last page (:885).
+            # So in a serialized stream the second scan (:835) never finds a run.
+            ffpb = h.scalars()[0]
+            for p in range(ffpb + 1, g.numpages):
+                assert h.pte(p)[0] > P, (step, p, ffpb)
         assert nulls > 0, "streams are sized to reach OOM"
+        assert nulls > wrapped
 
 
 def test_pagesize_chunk_leaks_one_slot(oracle_mod):
