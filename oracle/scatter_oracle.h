@@ -18,17 +18,21 @@
  * reference printed for 16,384 x malloc(16) on a 64 MiB heap; brute-force streams on tiny heaps checking
  * after every step that no allocations overlap, the wastefactor rule holds, a
  * page hands out its lowest free chunk, the count equals the live chunks and a
- * null means no page can take the request; the special cases of 0 bytes,
- * multi-page first fit from the top and the pagesize-chunk leak.
+ * null means no page can take the request (for multi-page requests: no run
+ * of free pages, or _firstFreePageBased wrapped below page 0), and every page
+ * above _firstFreePageBased belongs to a live multi-page run; the special
+ * cases of 0 bytes, multi-page first fit from the top and the pagesize-chunk
+ * leak.
  *
- * Parity unpinned: the order in which the region walk visits pages once a
- * size has pages in use -- the region fill-level increments and resets
- * (Scatter.hpp:612-617, 727-735), the access-block cursor (:620-629,
- * 733-734), _firstFreePageBased (:796-801, 885) and the switch to the second
- * pass (:553, 634).  These follow the reference by code reading only: the
- * reference holds no vectors for them and its CPU build needs Boost, which is
- * absent.  The GPU path matches this oracle on them bit-exactly, but no check
- * independent of the oracle fixes them.
+ * The stateful heuristics of the walk -- the region fill-level increment and
+ * its cap (Scatter.hpp:612-617), the reset at P/2/c (:727-735), the
+ * access-block cursor's advance, guard and atomicMin (:620-629, 733-734), the
+ * 3R/4 first-pass gate and the second pass (:553, 634) and
+ * _firstFreePageBased's CAS, atomicMax and uint32 wrap (:796-801, 885) -- are
+ * pinned by known-answer streams whose every offset, fill level and cursor
+ * value was derived by hand from Scatter.hpp (tests/oracle_kats.py).  Builds
+ * with ORC_MUTANT set break one heuristic each; they are test-only, and each
+ * fails its stream (tests/test_oracle_kats.py).
  *
  * Documented divergences from the reference (all cases where the reference is
  * undefined or unsafe; none is reachable by a valid workload otherwise):

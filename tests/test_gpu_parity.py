@@ -160,9 +160,10 @@ def test_concurrent_mixed_sizes_rounds(mmc, cuda, dist):
         r = a.verify_pattern(offs, sizes, salt=seed << 40)
         assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0, r
         # per-lane ScatterAlloc formats the first free page each lane meets, so
-        # at ~50% fill some requests come back null; they are counted, not errors
+        # at ~50% fill some requests come back null; they are counted, not
+        # errors (measured 6.2-7.3% on the MI355X)
         print(f"  nulls {r['nulls']} of {n}", flush=True)
-        assert r["nulls"] < n // 5, r
+        assert r["nulls"] < n // 10, r
         assert _check_no_overlap(torch, offs, sizes) == 0
         a.free_batch(offs)
         torch.cuda.synchronize()
