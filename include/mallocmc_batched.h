@@ -123,6 +123,24 @@ typedef struct mmc_step_desc {
  * form and the workgroup size. */
 int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tuning, void* stream);
 
+/* The step of mmc_multi_step over `heaps` directly, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc -- and so cannot build an mmc_multi group, whose heap
+ * table mmc_multi_create allocates.  The heap table and the tile cursor live
+ * in one fixed device array of the library instead (the table is written by
+ * kernels whose arguments carry the heap views, so the call can be
+ * captured); desc->d_tile_cursor is ignored.  Results are those of
+ * mmc_multi_step with the same heaps, descriptor and tuning.
+ * Limits and sharing: at most MMC_STATIC_MAX_HEAPS heaps (MMC_EINVAL beyond);
+ * every call on one device uses that device's one array, so calls that
+ * overlap on one device (on different streams, or from several host threads)
+ * race and must be serialized by the caller.  The first call on a device
+ * looks the array up and should happen outside a stream capture. */
+#define MMC_STATIC_MAX_HEAPS 256u
+int mmc_multi_step_static(mmc_heap* const* heaps, uint32_t nheaps, const mmc_step_desc* desc,
+                          const mmc_tuning* tuning, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,7 +39,7 @@ EXPORTED_SYMBOLS = [
 ]
 BATCHED_SYMBOLS = [
     "mmc_default_tuning", "mmc_wave_malloc_batch", "mmc_wave_free_batch", "mmc_multi_create",
-    "mmc_multi_destroy", "mmc_multi_step", "mmc_wave_fill_until_null",
+    "mmc_multi_destroy", "mmc_multi_step", "mmc_multi_step_static", "mmc_wave_fill_until_null",
     # include/mallocmc_serve.h, bound in mallocmc_amd/serve.py
     "mmc_serve_create", "mmc_serve_destroy", "mmc_serve_load", "mmc_serve_step", "mmc_serve_get_stats",
     "mmc_serve_timestamps", "mmc_serve_stamp", "mmc_serve_stamp_count",
@@ -237,6 +237,7 @@ def lib() -> ctypes.CDLL:
         "mmc_multi_create": ([P(vp), u32, P(vp)], i32),
         "mmc_multi_destroy": ([vp], i32),
         "mmc_multi_step": ([vp, P(MmcStepDesc), P(MmcTuning), vp], i32),
+        "mmc_multi_step_static": ([P(vp), u32, P(MmcStepDesc), P(MmcTuning), vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -434,6 +435,20 @@ class MultiHeap:
         t = (tuning or Tuning(persistent=True)).to_c()
         _check(lib().mmc_multi_step(self._m, ctypes.byref(desc), ctypes.byref(t), _stream_ptr(stream)),
                "mmc_multi_step")
+
+
+STATIC_MAX_HEAPS = 256  # MMC_STATIC_MAX_HEAPS
+
+
+def multi_step_static(allocators, desc: MmcStepDesc, tuning: Tuning | None = None, stream=None):
+    """One multi-heap step over `allocators` that allocates no device memory
+    (mmc_multi_step_static): the heap table and tile cursor live in the
+    library's fixed per-device array (at most STATIC_MAX_HEAPS heaps), so
+    desc.d_tile_cursor is not used.  Calls that overlap on one device race."""
+    heaps = (ctypes.c_void_p * len(allocators))(*[a.handle.value for a in allocators])
+    t = (tuning or Tuning(persistent=True)).to_c()
+    _check(lib().mmc_multi_step_static(heaps, len(allocators), ctypes.byref(desc), ctypes.byref(t),
+                                       _stream_ptr(stream)), "mmc_multi_step_static")
 
 
 def default_config() -> ScatterConfig:

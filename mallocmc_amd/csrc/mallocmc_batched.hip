@@ -8,6 +8,8 @@
 #include "mallocMC/detail/wave_alloc.hpp"
 #include "mmc_internal.hpp"
 
+#include <mutex>
+#include <string>
 #include <vector>
 
 using namespace mallocMC;
@@ -620,6 +622,96 @@ namespace
         const unsigned blocks = static_cast<unsigned>(cus) * wpc / 4u; // 4 waves per 256-thread block
         return blocks ? blocks : 1u;
     }
+
+    bool descUsable(const mmc_step_desc* d)
+    {
+        return d && d->d_slot_heap && d->d_tiles && d->d_size_table && d->d_size_cdf && d->d_offsets
+               && d->d_sizes && d->d_step && d->d_counters && d->n_sizes != 0u;
+    }
+
+    //! One multi-heap step over the device heap table `heaps`, in the form
+    //! the tuning selects.
+    void launchStep(const ScatterHeap* heaps, bool xm, const mmc_step_desc& d, const mmc_tuning* tuning,
+                    hipStream_t s)
+    {
+        const WaveTuning t = toTuning(tuning);
+        if(!tuning || tuning->persistent)
+        {
+            const unsigned blocks = persistentBlocks(tuning);
+            if(xm)
+                hipLaunchKernelGGL(multiStepKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, heaps, d, t);
+            else
+                hipLaunchKernelGGL(multiStepKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, heaps, d, t);
+            return;
+        }
+        const bool part = partitionOf(tuning);
+        switch(blockOf(tuning))
+        {
+        case 1024u:
+            launchMultiBlock<1024>(xm, heaps, d, t, part, s);
+            break;
+        case 512u:
+            launchMultiBlock<512>(xm, heaps, d, t, part, s);
+            break;
+        default:
+            launchMultiBlock<256>(xm, heaps, d, t, part, s);
+            break;
+        }
+    }
+
+    // ------------------------------------------------ mmc_multi_step_static
+    //! Device memory of the allocation-free step: the heap table
+    //! (MMC_STATIC_MAX_HEAPS views), then the two-word tile cursor on a
+    //! 256-byte line of its own.  48.25 KiB per device, held until the
+    //! process ends.
+    constexpr u64 kStaticTableBytes = MMC_STATIC_MAX_HEAPS * sizeof(ScatterHeap);
+    constexpr u64 kStaticScratchBytes = kStaticTableBytes + 256u;
+    static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+    //! Heap views one staging launch carries in its kernel argument (3 KiB).
+    constexpr u32 kViewsPerLaunch = 16u;
+    struct ViewChunk
+    {
+        ScatterHeap v[kViewsPerLaunch];
+    };
+    static_assert(sizeof(ViewChunk) <= 3200u, "a staging launch's argument must stay small");
+    static_assert(sizeof(ScatterHeap) % 8u == 0u, "views are copied as 64-bit words");
+
+    //! Copies `count` views from the kernel argument into the table; the
+    //! launch that stages the first views also zeroes the tile cursor.
+    __global__ __launch_bounds__(256) void stageViewsKernel(ViewChunk c, ScatterHeap* dst, u32 count, u32* cursor)
+    {
+        const u64* src = reinterpret_cast<const u64*>(&c.v[0]);
+        u64* out = reinterpret_cast<u64*>(dst);
+        const u32 words = count * static_cast<u32>(sizeof(ScatterHeap) / 8u);
+        for(u32 i = threadIdx.x; i < words; i += blockDim.x)
+            out[i] = src[i];
+        if(cursor != nullptr && threadIdx.x < 2u)
+            cursor[threadIdx.x] = 0u;
+    }
+
+    //! The array's device address per device, looked up once per device with
+    //! that device current (the symbol's host-side name is not a device
+    //! address).
+    std::mutex gScratchLock;
+    char* gScratchAddr[64] = {};
+
+    int staticScratch(int dev, char** out)
+    {
+        if(dev < 0 || dev >= 64)
+            return mmc::fail(MMC_EINVAL, "device ordinal out of range");
+        std::lock_guard<std::mutex> lock(gScratchLock);
+        if(gScratchAddr[dev] == nullptr)
+        {
+            void* p = nullptr;
+            const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if(err != hipSuccess || p == nullptr)
+                return mmc::hipFail(err, "hipGetSymbolAddress(scratch)");
+            gScratchAddr[dev] = static_cast<char*>(p);
+        }
+        *out = gScratchAddr[dev];
+        return MMC_OK;
+    }
 } // namespace
 
 namespace mmc
@@ -800,41 +892,62 @@ int mmc_multi_destroy(mmc_multi* m)
 
 int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tuning, void* stream)
 {
-    if(!m || !desc || !desc->d_slot_heap || !desc->d_tiles || !desc->d_size_table || !desc->d_size_cdf
-       || !desc->d_offsets || !desc->d_sizes || !desc->d_step || !desc->d_counters || !desc->d_tile_cursor
-       || desc->n_sizes == 0)
+    if(!m || !descUsable(desc) || !desc->d_tile_cursor)
         return mmc::fail(MMC_EINVAL, "bad step descriptor");
     if(desc->n_tiles == 0)
         return MMC_OK;
-    const WaveTuning t = toTuning(tuning);
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const bool xm = m->distribution == MMC_DIST_XMALLOCSIMD;
-    if(!tuning || tuning->persistent)
-    {
-        const unsigned blocks = persistentBlocks(tuning);
-        if(xm)
-            hipLaunchKernelGGL(multiStepKernel<XMallocSIMD64>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc, t);
-        else
-            hipLaunchKernelGGL(multiStepKernel<NoopDistribution>, dim3(blocks), dim3(256), 0, s, m->dHeaps, *desc,
-                               t);
-    }
-    else
-    {
-        const bool part = partitionOf(tuning);
-        switch(blockOf(tuning))
-        {
-        case 1024u:
-            launchMultiBlock<1024>(xm, m->dHeaps, *desc, t, part, s);
-            break;
-        case 512u:
-            launchMultiBlock<512>(xm, m->dHeaps, *desc, t, part, s);
-            break;
-        default:
-            launchMultiBlock<256>(xm, m->dHeaps, *desc, t, part, s);
-            break;
-        }
-    }
+    launchStep(m->dHeaps, m->distribution == MMC_DIST_XMALLOCSIMD, *desc, tuning, static_cast<hipStream_t>(stream));
     const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "multi step launch");
+}
+
+int mmc_multi_step_static(mmc_heap* const* heaps, uint32_t nheaps, const mmc_step_desc* desc,
+                          const mmc_tuning* tuning, void* stream)
+{
+    if(!heaps || !descUsable(desc))
+        return mmc::fail(MMC_EINVAL, "bad argument");
+    if(nheaps == 0u || nheaps > MMC_STATIC_MAX_HEAPS)
+        return mmc::fail(MMC_EINVAL, "mmc_multi_step_static serves 1 to " + std::to_string(MMC_STATIC_MAX_HEAPS)
+                                         + " heaps, got " + std::to_string(nheaps));
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if(err != hipSuccess)
+        return mmc::hipFail(err, "hipGetDevice");
+    const u32 dist = heaps[0] ? heaps[0]->cfg.distribution : 0u;
+    for(u32 i = 0; i < nheaps; ++i)
+    {
+        if(!heaps[i])
+            return mmc::fail(MMC_EINVAL, "null heap in group");
+        if(heaps[i]->cfg.distribution != dist)
+            return mmc::fail(MMC_EINVAL, "heaps of one step must share the distribution policy");
+        if(heaps[i]->device != dev)
+            return mmc::fail(MMC_EINVAL, "every heap must live on the current device");
+    }
+    if(desc->n_tiles == 0)
+        return MMC_OK;
+    char* base = nullptr;
+    const int rc = staticScratch(dev, &base);
+    if(rc != MMC_OK)
+        return rc;
+    ScatterHeap* table = reinterpret_cast<ScatterHeap*>(base);
+    u32* cursor = reinterpret_cast<u32*>(base + kStaticTableBytes);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // the array keeps the last call's table: rewrite this call's views, and
+    // start the tile cursor from zero (kernels, not memsets, so a capture
+    // records kernel nodes only)
+    for(u32 first = 0; first < nheaps; first += kViewsPerLaunch)
+    {
+        ViewChunk c;
+        const u32 n = nheaps - first < kViewsPerLaunch ? nheaps - first : kViewsPerLaunch;
+        for(u32 i = 0; i < n; ++i)
+            c.v[i] = heaps[first + i]->view;
+        hipLaunchKernelGGL(stageViewsKernel, dim3(1), dim3(256), 0, s, c, table + first, n,
+                           first == 0u ? cursor : nullptr);
+    }
+    mmc_step_desc d = *desc;
+    d.d_tile_cursor = cursor;
+    launchStep(table, dist == MMC_DIST_XMALLOCSIMD, d, tuning, s);
+    err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "multi step launch");
 }
 
