@@ -62,10 +62,12 @@ def _no_overlap(torch, offs, sizes):
 
 def run_default(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles: int = 5, pagesize: int = 65536,
                 regionsize: int = 16, tuning: mmc.Tuning | None = None, log=None, cap: int | None = None,
-                first_null: bool = True) -> dict:
+                first_null: bool = True, verify_snapshots: bool = False) -> dict:
     """first_null: every lane stops at the first null any lane sees (the
     heap's fill at its first failure); else each lane allocates until its own
-    null (verify_heap's AllocAll)."""
+    null (verify_heap's AllocAll).  verify_snapshots: after every fill, also
+    check a snapshot of the heap's metadata against the complete list of live
+    allocations (mallocmc_amd/snapshot.py)."""
     import torch
     L = _bind()
     tuning = tuning or mmc.Tuning(aggregation=64, class_split=2)
@@ -102,6 +104,11 @@ def run_default(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles
         alloc.write_pattern(live_o, live_s, salt=cyc << 48)
         v = alloc.verify_pattern(live_o, live_s, salt=cyc << 48)
         cyc_ok = overlap_ok and v["mismatches"] == 0 and v["misaligned"] == 0 and v["outside"] == 0 and v["nulls"] == 0
+        snap_report = None
+        if verify_snapshots:
+            from mallocmc_amd import snapshot
+            snap_report = snapshot.verify(snapshot.take(alloc), live_o, live_s, complete=True)
+            cyc_ok = cyc_ok and snap_report["ok"]
         ok = ok and cyc_ok
         live_bytes = int((((live_s.to(torch.int64) + 15) // 16) * 16).sum().item())
         slots = [alloc.getAvailableSlots(s) for s in probe]
@@ -118,6 +125,8 @@ def run_default(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles
         count.fill_(m)
         rec = dict(cycle=cyc, new_allocations=n - before, live_allocations=n, fill_fraction=live_bytes / page_bytes,
                    fill_s=t_fill, free_s=t_free, invariants_ok=bool(cyc_ok), slots_after_fill=slots)
+        if snap_report is not None:
+            rec["snapshot"] = {k: snap_report[k] for k in ("ok", "errors", "used_chunks", "free_pages", "message")}
         results.append(rec)
         if log:
             log(f"oom stress cycle {rec}")

@@ -161,3 +161,17 @@ def test_fastdiv_matches_division(mmc):
     bad = ctypes.c_uint64(123)
     assert mmc.lib().mmc_selftest_fastdiv(7, 1_000_000, ctypes.byref(bad)) == 0
     assert bad.value == 0
+
+
+def test_serve_checksums_vectorized_equal_closed_form():
+    """The chunked torch form used to check every served request equals the
+    per-request closed form, across chunk boundaries and at the length cap."""
+    import numpy as np
+    from mallocmc_amd import serve
+    rng = np.random.default_rng(2)
+    ids = rng.integers(0, 2 ** 20, 12)
+    lens = rng.integers(1, 200, 12)
+    lens[0], lens[1] = 8192, 1
+    got = serve.expected_checksums(ids, lens, device="cpu", chunk_terms=1 << 14)
+    want = np.array([serve.expected_checksum(int(i), int(n)) for i, n in zip(ids, lens)], dtype=np.uint64)
+    np.testing.assert_array_equal(got, want)
