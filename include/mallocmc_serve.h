@@ -37,7 +37,7 @@ typedef struct mmc_serve_config {
     uint32_t max_slots;        /* concurrent requests (multiple of 64) */
     uint32_t block_bytes;      /* bytes per paged-buffer block (e.g. 4096) */
     uint32_t tokens_per_block; /* tokens per block (e.g. 16) */
-    uint32_t token_bytes;      /* bytes per token, multiple of 4 (e.g. 256) */
+    uint32_t token_bytes;      /* bytes per token, multiple of 16 (tokens are written as 16-byte words; e.g. 256) */
     uint32_t max_tokens;       /* longest request (e.g. 8192) */
     uint32_t reserved;
     uint64_t budget_bytes;     /* admission budget in heap bytes */
@@ -66,6 +66,10 @@ static inline uint32_t mmc_serve_word(uint32_t r, uint32_t t, uint32_t w)
     return x;
 }
 
+/* Fails with MMC_EINVAL unless max_slots is a multiple of 64, token_bytes a
+ * multiple of 16, tokens_per_block * token_bytes <= block_bytes, and
+ * 64 * ceil(max_tokens / tokens_per_block) * block_bytes < 2^32 (the
+ * admission adds up 64 requests' projected bytes in 32 bits). */
 int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** out);
 int mmc_serve_destroy(mmc_serve* s);
 
@@ -77,7 +81,10 @@ int mmc_serve_destroy(mmc_serve* s);
  * k = 0 .. length*W - 1, read back from the paged buffer.
  * d_admit_step (n entries, optional) receives its admission step.  Resets the
  * loop (request slots, counters, step 0); the heap must hold no blocks of an
- * earlier trace. */
+ * earlier trace.  Checks the trace on the device and blocks on `stream` for
+ * the result: a length outside 1..max_tokens or an arrival step below its
+ * predecessor's fails with MMC_EINVAL naming the first such request, and
+ * nothing is admitted. */
 int mmc_serve_load(mmc_serve* s, const uint32_t* d_arrival, const uint32_t* d_length, const uint32_t* d_ids,
                    uint32_t n, uint64_t* d_checksums, uint32_t* d_admit_step, void* stream);
 

@@ -5,6 +5,8 @@
 #include "mallocMC/detail/wave_alloc.hpp"
 #include "mmc_internal.hpp"
 
+#include <string>
+
 using namespace mallocMC;
 using namespace mallocMC::detail;
 
@@ -36,6 +38,8 @@ namespace
         u64 counters[cCount];
         u32 nStamps;  // device-clock stamps recorded since load
         u32 parity;   // the done list this step's decode filled (completion reads it)
+        u32 firstBad; // index of the first request a load rejected (UINT32_MAX: none)
+        u32 pad;
     };
 
     constexpr u32 kTokensPerChunk = 64u;
@@ -468,8 +472,24 @@ namespace
             v.sc->nChunks[1] = 0u;
             v.sc->parity = 0u;
             v.sc->nStamps = 0u;
+            v.sc->firstBad = 0xffffffffu;
             for(u32 c = 0; c < cCount; ++c)
                 v.sc->counters[c] = 0u;
+        }
+    }
+
+    //! Finds the first request with a length outside 1..maxTokens or an
+    //! arrival step below its predecessor's (the loop's preconditions).
+    __global__ void validateKernel(ServeView v, u32 maxTokens)
+    {
+        const u32 tid = blockIdx.x * blockDim.x + threadIdx.x;
+        const u32 stride = gridDim.x * blockDim.x;
+        for(u32 i = tid; i < v.nRequests; i += stride)
+        {
+            const u32 len = v.length[i];
+            const bool bad = len == 0u || len > maxTokens || (i > 0u && v.arrival[i] < v.arrival[i - 1u]);
+            if(bad)
+                atom::min(&v.sc->firstBad, i);
         }
     }
 
@@ -508,10 +528,13 @@ int mmc_serve_create(mmc_heap* heap, const mmc_serve_config* cfg, mmc_serve** ou
        || cfg->token_bytes == 0 || cfg->token_bytes % 16u != 0 || cfg->max_tokens == 0
        || static_cast<u64>(cfg->tokens_per_block) * cfg->token_bytes > cfg->block_bytes)
         return mmc::fail(MMC_EINVAL, "bad serving configuration");
+    const u32 maxBlocks = divup(cfg->max_tokens, cfg->tokens_per_block);
+    if(64u * static_cast<u64>(maxBlocks) * cfg->block_bytes > 0xffffffffull)
+        return mmc::fail(MMC_EINVAL, "64 x ceil(max_tokens / tokens_per_block) x block_bytes must stay below 2^32: "
+                                     "admission adds 64 requests' projected bytes in 32 bits");
     mmc_serve* s = new mmc_serve();
     s->heap = heap;
     s->cfg = *cfg;
-    const u32 maxBlocks = divup(cfg->max_tokens, cfg->tokens_per_block);
     const u64 n = cfg->max_slots;
     const u64 bytes = sizeof(ServeScalars) + 4u * n * 10u + 8u * n * maxBlocks + 256u;
     hipError_t err = hipMalloc(&s->arena, bytes);
@@ -572,9 +595,28 @@ int mmc_serve_load(mmc_serve* s, const uint32_t* d_arrival, const uint32_t* d_le
     s->view.checksums = d_checksums;
     s->view.admitStep = d_admit_step;
     s->view.nRequests = n;
-    hipLaunchKernelGGL(resetKernel, dim3(64), dim3(256), 0, static_cast<hipStream_t>(stream), s->view);
-    hipLaunchKernelGGL(firstAdmitKernel, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), s->view);
-    const hipError_t err = hipGetLastError();
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(resetKernel, dim3(64), dim3(256), 0, st, s->view);
+    if(n)
+        hipLaunchKernelGGL(validateKernel, dim3(mmc::gridFor(n, 256) < 1024u ? mmc::gridFor(n, 256) : 1024u),
+                           dim3(256), 0, st, s->view, s->cfg.max_tokens);
+    u32 firstBad = 0xffffffffu;
+    hipError_t err = hipGetLastError();
+    if(err == hipSuccess)
+        err = hipMemcpyAsync(&firstBad, &s->view.sc->firstBad, sizeof(u32), hipMemcpyDeviceToHost, st);
+    if(err == hipSuccess)
+        err = hipStreamSynchronize(st);
+    if(err != hipSuccess)
+        return mmc::hipFail(err, "serve load");
+    if(firstBad != 0xffffffffu)
+    {
+        s->view.nRequests = 0;
+        return mmc::fail(MMC_EINVAL, "request " + std::to_string(firstBad) + ": length must be 1.." +
+                                         std::to_string(s->cfg.max_tokens) +
+                                         " tokens and arrival steps must not decrease");
+    }
+    hipLaunchKernelGGL(firstAdmitKernel, dim3(1), dim3(64), 0, st, s->view);
+    err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve reset launch");
 }
 

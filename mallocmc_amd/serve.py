@@ -81,6 +81,23 @@ def make_trace(seed: int = 5, rate_per_s: float = 2000.0, step_s: float = 0.01, 
     return Trace(arrival, length, np.arange(len(arrival), dtype=np.uint32))
 
 
+def validate_trace(trace: Trace, max_tokens: int) -> None:
+    """The loop's preconditions (include/mallocmc_serve.h, mmc_serve_load):
+    every length in 1..max_tokens and arrival steps that never decrease.
+    mmc_serve_load checks the same on the device; this names the first
+    offending request before anything is copied."""
+    length = np.asarray(trace.length)
+    arrival = np.asarray(trace.arrival).astype(np.int64)
+    if not (len(length) == len(arrival) == len(trace.ids)):
+        raise ValueError("trace arrays differ in length")
+    bad = np.nonzero((length < 1) | (length > max_tokens))[0]
+    if len(bad):
+        raise ValueError(f"request {int(bad[0])}: length {int(length[bad[0]])} is outside 1..{max_tokens}")
+    down = np.nonzero(np.diff(arrival) < 0)[0]
+    if len(down):
+        raise ValueError(f"request {int(down[0]) + 1}: arrival steps must not decrease")
+
+
 def payload_words(rid: int, t: np.ndarray, w: np.ndarray) -> np.ndarray:
     """mmc_serve_word (include/mallocmc_serve.h) over arrays, uint32 arithmetic."""
     with np.errstate(over="ignore"):
@@ -171,8 +188,12 @@ class ServeLoop:
         except Exception:
             pass
 
-    def load(self, trace: Trace, stream=None):
+    def load(self, trace: Trace, stream=None, check: bool = True):
+        """Copies the trace to the device and resets the loop; `check`
+        validates it on the host first (the library checks it again)."""
         import torch
+        if check:
+            validate_trace(trace, self.cfg.max_tokens)
         dev = "cuda"
         self.trace = trace
         self.d_arrival = torch.from_numpy(trace.arrival.astype(np.int32)).to(dev)

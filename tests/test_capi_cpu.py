@@ -175,3 +175,18 @@ def test_serve_checksums_vectorized_equal_closed_form():
     got = serve.expected_checksums(ids, lens, device="cpu", chunk_terms=1 << 14)
     want = np.array([serve.expected_checksum(int(i), int(n)) for i, n in zip(ids, lens)], dtype=np.uint64)
     np.testing.assert_array_equal(got, want)
+
+
+def test_serve_trace_validation():
+    """Zero or over-long lengths and decreasing arrivals are rejected before a
+    trace reaches the device (mmc_serve_load checks them again there)."""
+    import numpy as np
+    from mallocmc_amd import serve
+    ok = serve.Trace(np.array([0, 0, 1], np.uint32), np.array([1, 8192, 5], np.uint32), np.arange(3, dtype=np.uint32))
+    serve.validate_trace(ok, 8192)
+    for arrival, length, msg in (([0, 0, 1], [1, 0, 5], "request 1: length 0"),
+                                 ([0, 0, 1], [1, 8193, 5], "request 1: length 8193"),
+                                 ([0, 2, 1], [1, 1, 1], "request 2: arrival")):
+        t = serve.Trace(np.array(arrival, np.uint32), np.array(length, np.uint32), np.arange(3, dtype=np.uint32))
+        with pytest.raises(ValueError, match=msg):
+            serve.validate_trace(t, 8192)

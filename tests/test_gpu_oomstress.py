@@ -10,16 +10,28 @@ pytestmark = pytest.mark.gpu
 GiB = 1024 ** 3
 
 
-@pytest.mark.parametrize("split,first_null", [(0, False), (2, False), (2, True)])
-def test_oom_stress_cycles(mmc, cuda, split, first_null):
+@pytest.mark.parametrize("split,first_null,block_threads,no_partition", [
+    (0, False, 0, False), (2, False, 0, False), (2, True, 0, False),
+    (2, True, 256, False), (2, False, 256, False), (2, True, 512, False), (2, False, 512, False),
+    (2, True, 0, True), (2, False, 0, True),
+])
+def test_oom_stress_cycles(mmc, cuda, split, first_null, block_threads, no_partition):
+    """Every form of the fill: workgroups of 1,024, 512 or 256 lanes that
+    reorder a round of requests by size (a lane's null travels back to it
+    through LDS), and the arrival-order kernel; stopping at the first null or
+    each lane at its own.  Invariants and a snapshot check against the live
+    list after every fill, slot counts restored at the end."""
     from mallocmc_amd import oomstress
-    r = oomstress.run_default(heap_bytes=1 * GiB, lanes=65536, cycles=3,
-                              tuning=mmc.Tuning(aggregation=64, class_split=split), log=print,
-                              first_null=first_null)
+    tun = mmc.Tuning(aggregation=64, class_split=split, block_threads=block_threads, no_partition=no_partition)
+    r = oomstress.run_default(heap_bytes=1 * GiB, lanes=65536, cycles=3, tuning=tun, log=print,
+                              first_null=first_null, verify_snapshots=True)
     assert r["invariants_ok"], r
     assert r["slots_restored"], r
     for c in r["cycles"]:
         assert c["fill_fraction"] > 0.5, c
+        assert c["snapshot"]["ok"], c["snapshot"]
+        if first_null:
+            assert c["slots_after_fill"][3] == 0, c  # no free page at the first null
 
 
 def test_oom_first_null_fill_close_to_reference_protocol(mmc, cuda, oracle_mod):

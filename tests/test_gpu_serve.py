@@ -75,3 +75,33 @@ def test_serve_alone_equals_mix_under_admission_pressure(mmc, cuda):
                                                                                    int(trace.length[i]))
     loop.close()
     a.close()
+
+
+def test_serve_rejects_overflowing_config_and_bad_traces(mmc, cuda):
+    """mmc_serve_create refuses a configuration whose 64-request admission sum
+    could pass 2^32; mmc_serve_load refuses (on the device, without the host
+    check) a trace with a zero or over-long length or a decreasing arrival,
+    names the request and admits nothing.  A good trace then runs."""
+    import ctypes
+    from mallocmc_amd import serve
+    a = _alloc(mmc)
+    L = serve._bind()
+    cfg = serve.MmcServeConfig(64, 1 << 20, 16, 256, 8192, 0, 64 * MiB)  # 64 x 512 x 1 MiB > 2^32
+    h = ctypes.c_void_p()
+    assert L.mmc_serve_create(a.handle, ctypes.byref(cfg), ctypes.byref(h)) == -1
+    assert b"2^32" in mmc.lib().mmc_last_error()
+    loop = serve.ServeLoop(a, max_slots=64, max_tokens=100, budget_bytes=64 * MiB)
+    ids = np.arange(4, dtype=np.uint32)
+    for arrival, length, first in (([0, 0, 1, 1], [5, 0, 5, 5], 1), ([0, 0, 1, 1], [5, 5, 5, 101], 3),
+                                   ([0, 2, 1, 3], [5, 5, 5, 5], 2)):
+        t = serve.Trace(np.array(arrival, np.uint32), np.array(length, np.uint32), ids)
+        with pytest.raises(mmc.MallocMCError, match=f"request {first}:"):
+            loop.load(t, check=False)
+        assert loop.stats()["admitted"] == 0
+    good = serve.Trace(np.array([0, 0, 1, 1], np.uint32), np.array([5, 20, 1, 100], np.uint32), ids)
+    st = _run_all(loop, good, chunk=64)
+    assert st["completed"] == 4 and st["nulls"] == 0
+    cs = loop.checksums()
+    assert all(int(cs[i]) == serve.expected_checksum(int(ids[i]), int(good.length[i])) for i in range(4))
+    loop.close()
+    a.close()
