@@ -46,7 +46,17 @@ HBM_PEAK_GBS = 8000.0
 # reference protocol's three 4-byte metadata read-modify-writes (chunk size CAS,
 # count add, bitmask or) at 8 B each
 MALLOC_BYTES_PER_REQUEST = 4 + 8 + 3 * 8
-MALLOC_RMW_PER_REQUEST = 3
+# Autotuner points within TUNE_TIE of the fastest count as equal (run-to-run
+# noise is of that order); the stored point, the one the autotuner picked on
+# this workload, wins such a tie, so the counter profiles under profiles/
+# (taken with --no-tune) describe the point this run measures.
+TUNE_TIE = 0.03
+COUNTERS_PROFILE = os.path.join(HERE, "profiles", "batched_mixed1M_counters.json")
+
+
+def stored_point():
+    from mallocmc_amd import autotune
+    return autotune.TunePoint(pagesize=65536, regionsize=16, aggregation=64, persistent=False, xcd_slices=8)
 
 
 def log(*a):
@@ -134,23 +144,140 @@ def measure_batched_reference_geometry(torch, mmc, batches, steps):
                         "path": "wave-cooperative", "launch": "eager, one malloc + one free launch per step"})
 
 
+def cpu_threads():
+    """Thread counts the CPU baseline is run at: the OpenMP share this host
+    gives the process (OMP_NUM_THREADS) and every CPU the process may run on."""
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return sorted({t for t in (share, avail) if t > 0})
+
+
+def cpu_run(orc, sizes_np, cfg, heap, threads):
+    """One concurrent malloc / write / verify / free pass of the oracle in the
+    reference's CpuOmp2Blocks model with its 16-way mutex atomics."""
+    h = orc.OracleHeap(cfg, heap)
+    h.set_atomic_mode(orc.ATOMIC_MUTEX16)
+    res, _ = h.bench_concurrent(sizes_np, nthreads=threads)
+    h.close()
+    return 2 * len(sizes_np) / (res.t_malloc + res.t_free), res
+
+
 def cpu_baseline(sizes_np):
-    """The oracle (reference CpuOmp2Blocks model, 16-way mutex atomics) on this host."""
+    """The oracle (reference CpuOmp2Blocks model, 16-way mutex atomics) on this
+    host's cores, at every thread count of cpu_threads(); the fastest is the
+    reported value."""
     from oracle import oracle as orc
     orc.lib()
     n = len(sizes_np)
-    cores = min(16, os.cpu_count() or 1)
     cfg = orc.make_config(pagesize=16384, accessblocksize=2 * GiB, resetfreedpages=True)
-    h = orc.OracleHeap(cfg, HEAP)
-    h.set_atomic_mode(orc.ATOMIC_MUTEX16)
-    res, _ = h.bench_concurrent(sizes_np, nthreads=cores)
-    h.close()
-    ops = 2 * n / (res.t_malloc + res.t_free)
+    sweep = {}
+    best = None
+    for t in cpu_threads():
+        ops, res = cpu_run(orc, sizes_np, cfg, HEAP, t)
+        sweep[str(res.threads_used)] = ops
+        if best is None or ops > best[0]:
+            best = (ops, res)
+    ops, res = best
     return {"value": ops, "unit": "mallocs+frees/s", "cores": res.threads_used, "kind": "port",
             "sample": f"mixed-size batch, seed 1: {n} requests, 4 GiB heap, pagesize 16 KiB, one malloc and one free "
                       f"each, blocks x 1 thread over OpenMP (CpuOmp2Blocks), 16-way mutex atomics",
+            "threads_sweep": sweep, "host_cpus": os.cpu_count(),
             "nulls": int(res.nulls), "mismatches": int(res.mismatches),
             "t_malloc_s": res.t_malloc, "t_free_s": res.t_free}
+
+
+def config1(torch, mmc, reps=20):
+    """The reference's own small case: heap 64 MiB, pagesize 4 KiB, access
+    blocks of 2 MiB, regionsize 16, resetfreedpages; 64 x 256 lanes each
+    malloc 16 B, write their id, verify, free.  GPU: the per-lane form and
+    the wave form, malloc + free launches timed with HIP events (median of
+    `reps`); CPU: the oracle in the CpuOmp2Blocks model (median of `reps`)."""
+    from oracle import oracle as orc
+    MiB = 1024 ** 2
+    n = 64 * 256
+    cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=2 * MiB, regionsize=16, resetfreedpages=True)
+    a = mmc.ScatterAllocator(cfg, 64 * MiB)
+    s0 = a.getAvailableSlots(16)
+    sizes = torch.full((n,), 16, dtype=torch.int32, device="cuda")
+    offs = torch.empty(n, dtype=torch.int64, device="cuda")
+    out = {"workload": "64 MiB heap, pagesize 4 KiB, 64 x 256 lanes x malloc(16), id written and verified, free"}
+    ok = True
+    for form, mal, fre in (("gpu_plain", a.malloc_batch, a.free_batch),
+                           ("gpu_batched", a.wave_malloc_batch, a.wave_free_batch)):
+        times = []
+        for r in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            mal(sizes, offs)
+            fre(offs)
+            e1.record()
+            torch.cuda.synchronize()
+            if r == 0:
+                mal(sizes, offs)
+                a.write_pattern(offs, sizes, salt=0)
+                v = a.verify_pattern(offs, sizes, salt=0)
+                ok = ok and v["mismatches"] == 0 and v["nulls"] == 0 and v["outside"] == 0
+                fre(offs)
+                torch.cuda.synchronize()
+            else:
+                times.append(e0.elapsed_time(e1))
+        out[form] = 2 * n / (float(np.median(times)) / 1e3)
+    ok = ok and a.getAvailableSlots(16) == s0
+    a.close()
+    orc.lib()
+    ocfg = orc.make_config(pagesize=4096, accessblocksize=2 * MiB, resetfreedpages=True)
+    sz = np.full(n, 16, dtype=np.uint32)
+    cpu = {}
+    for t in cpu_threads():
+        vals = [cpu_run(orc, sz, ocfg, 64 * MiB, t)[0] for _ in range(reps)]
+        cpu[str(t)] = float(np.median(vals))
+    out["cpu"] = max(cpu.values())
+    out["cpu_threads_sweep"] = cpu
+    out["invariants_ok"] = bool(ok)
+    return out
+
+
+def atomic_roofline_of(point, mal_avg_ms, ceiling):
+    """Atomic side of the malloc kernel's roofline from counters: the L2
+    atomic requests one launch issues (TCC_ATOMIC_sum per dispatch of a
+    committed counter pass at this point) over the launch time measured here,
+    against the atomic microbenchmark's L2 atomic requests per second (its
+    lane-atomics per second measured here, times the requests per lane-atomic
+    the same counter pass counted for it)."""
+    run = counter_run(point)
+    if run is None:
+        return {"bound": "atomic", "note": f"no counter pass at {point_key(point)} in {PROFILE_NAME}"}
+    per_launch = run["waveMallocKernel"]["TCC_ATOMIC_sum"]
+    per_op = run["atomicsKernel"]["TCC_ATOMIC_per_lane_op"]
+    achieved = per_launch / (mal_avg_ms / 1e3)
+    peak = ceiling * per_op
+    return {
+        "bound": "atomic", "achieved": achieved, "peak": peak, "unit": "L2 atomic requests/s",
+        "frac": achieved / peak, "kernel": "waveMallocKernel",
+        "atomics_per_launch": per_launch, "atomics_per_request": per_launch / N_LANES,
+        "atomic_instructions_per_launch": run["waveMallocKernel"].get("TA_FLAT_ATOMIC_WAVEFRONTS_sum"),
+        "free_kernel_atomics_per_launch": run.get("waveFreeKernel", {}).get("TCC_ATOMIC_sum"),
+        "profile": PROFILE_NAME, "profile_point": point_key(point),
+    }
+
+
+PROFILE_NAME = os.path.relpath(COUNTERS_PROFILE, HERE)
+
+
+def counter_run(point):
+    """The committed counter passes at `point` (profiles/batched_mixed1M_counters.json), or None."""
+    if not os.path.exists(COUNTERS_PROFILE):
+        return None
+    with open(COUNTERS_PROFILE) as f:
+        return json.load(f).get("runs", {}).get(point_key(point))
+
+
+def point_key(point):
+    """Key of a tuning point in the counter profiles."""
+    return f"P{point.pagesize}_R{point.regionsize}_A{point.aggregation}_X{point.xcd_slices}"
 
 
 def main():
@@ -165,6 +292,12 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-heap section")
     ap.add_argument("--no-oom", action="store_true", help="skip the OOM-stress section")
+    ap.add_argument("--no-refgeom", action="store_true",
+                    help="skip the batched form at the plain form's geometry (profiling runs)")
+    ap.add_argument("--xcd-slices", type=int, default=None,
+                    help="with --no-tune: XCD slices of the stored point (for counter comparisons)")
+    ap.add_argument("--regionsize", type=int, default=None,
+                    help="with --no-tune: regionsize of the stored point (for counter comparisons)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -176,6 +309,7 @@ def main():
     import mallocmc_amd as mmc
     from mallocmc_amd import autotune
     from mallocmc_amd import workloads as wl
+    STORED_POINT = stored_point()
 
     mmc.lib()
     torch.cuda.init()
@@ -191,8 +325,8 @@ def main():
     # pages are released through the chunked path instead of leaking a slot)
     base = mmc.ScatterConfig(accessblocksize=2 * GiB, resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
     if args.no_tune:
-        # the point the autotuner picks on this workload (pagesize 64 KiB, regionsize 32)
-        best = autotune.TunePoint(pagesize=65536, regionsize=32, aggregation=64, persistent=False, xcd_slices=0)
+        over = {k: v for k, v in (("xcd_slices", args.xcd_slices), ("regionsize", args.regionsize)) if v is not None}
+        best = autotune.TunePoint(**{**STORED_POINT.__dict__, **over})
         tuned_points = 0
     else:
         t0 = time.time()
@@ -200,7 +334,8 @@ def main():
         # requests in arrival order, without the block partition (three to
         # four times slower); the batched form sweeps the rest
         space = dict(autotune.DEFAULT_SPACE, persistent=(False,))
-        best, points = autotune.tune(base, HEAP, batches, space=space, log=log)
+        best, points = autotune.tune(base, HEAP, batches, space=space, log=log, prefer=STORED_POINT,
+                                     tie=TUNE_TIE)
         tuned_points = len(points)
         log(f"autotune: {len(points)} points in {time.time() - t0:.1f}s, best {best}")
     cfg = best.config(base)
@@ -265,24 +400,11 @@ def main():
     alloc.close()
     del graphs
     mal_avg_ms = float(np.mean(mal_ms[1:]))
-    # HBM bytes per malloc launch from the committed counter passes (see the file)
-    traffic = None
-    tpath = os.path.join(HERE, "profiles", "batched_mixed1M_hbm_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("waveMallocKernel<NoopDistribution>", {}).get("hbm_bytes_per_launch")
+    # HBM bytes per malloc launch from the committed counter passes at this point
+    run = counter_run(best)
+    traffic = run["waveMallocKernel"].get("hbm_bytes_per_launch") if run else None
     achieved_gbs = N_LANES * MALLOC_BYTES_PER_REQUEST / (mal_avg_ms / 1e3) / 1e9
-    atom_rate = N_LANES * MALLOC_RMW_PER_REQUEST / (mal_avg_ms / 1e3)
-    # the reference protocol's read-modify-writes per malloc+free pair, uncontended,
-    # with resetfreedpages: malloc = chunk-size CAS + count add + mask or + chunk-size
-    # re-CAS (Scatter.hpp:469-517, 589-591); free = mask and (+ segment-hint and on
-    # a hierarchical page) + failed count CAS + count sub (Scatter.hpp:646-716):
-    # 8 for chunks up to the hierarchy threshold, 7 above it
-    ht = (best.pagesize - 8) // 33
-    sz = np.concatenate([b.cpu().numpy() for b in batches])
-    frac_h = float(np.mean(((sz + 15) // 16 * 16) <= ht))
-    rmw_pair = 8.0 * frac_h + 7.0 * (1.0 - frac_h)
-    pair_rate = value / 2.0 * rmw_pair
+    atomic_roofline = atomic_roofline_of(best, mal_avg_ms, ceiling)
 
     # copy kernel of known size: calibrates the HBM byte counters of a profile run
     copy_bytes = 256 * 1024 * 1024
@@ -294,8 +416,10 @@ def main():
     torch.cuda.synchronize()
     del src, dst
 
-    ref_geom = measure_batched_reference_geometry(torch, mmc, batches, args.steps)
-    log(f"batched at the reference geometry: {ref_geom}")
+    ref_geom = None
+    if not args.no_refgeom:
+        ref_geom = measure_batched_reference_geometry(torch, mmc, batches, args.steps)
+        log(f"batched at the reference geometry: {ref_geom}")
 
     plain = None
     if not args.no_plain:
@@ -324,6 +448,9 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(sizes_np[0])
         log(f"cpu baseline: {cpu}")
+    c1 = None if args.no_cpu else config1(torch, mmc)
+    if c1:
+        log(f"config 1: {c1}")
 
     out = {
         "metric": METRIC,
@@ -359,17 +486,9 @@ def main():
             "kernel": "waveMallocKernel", "avg_launch_ms": mal_avg_ms,
             "bytes_per_request": MALLOC_BYTES_PER_REQUEST,
         },
-        "atomic_roofline": {
-            "achieved": atom_rate, "peak": ceiling, "unit": "reference-protocol RMW/s",
-            "frac": atom_rate / ceiling, "rmw_per_request": MALLOC_RMW_PER_REQUEST,
-            "per_pair": {
-                "achieved": pair_rate, "frac": pair_rate / ceiling, "rmw_per_pair": rmw_pair,
-                "hierarchical_fraction": frac_h,
-                "note": "whole step (malloc+free pairs/s) x the reference protocol's RMW per pair; "
-                        "the wave path issues fewer RMW than that, so frac may exceed 1",
-            },
-        },
+        "atomic_roofline": atomic_roofline,
         "cpu_baseline": cpu,
+        "config1": c1,
         "serve": serve,
         "batched_multi_heap": multi,
         "oom_stress": oom,

@@ -46,41 +46,58 @@ DEFAULT_SPACE = dict(
 )
 
 
-def measure(alloc: mmc.ScatterAllocator, size_batches, tuning: mmc.Tuning, rounds: int = 2):
-    """(mallocs+frees)/s over `rounds` passes of every batch, invariant check on the last."""
+def measure(alloc: mmc.ScatterAllocator, size_batches, tuning: mmc.Tuning, rounds: int = 2, timed: int = 1):
+    """(mallocs+frees)/s: `rounds` passes over every batch, the last `timed`
+    of them timed (the median pass rate is returned); the invariants are
+    checked on the last pass."""
+    import statistics
+
     import torch
-    n_ops = 0
-    total_ms = 0.0
     nulls = 0
     ok = True
+    rates = []
     for r in range(rounds):
+        n_ops = 0
+        total_ms = 0.0
         for sizes in size_batches:
             offs = torch.empty(sizes.numel(), dtype=torch.int64, device=sizes.device)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             alloc.wave_malloc_batch(sizes, offs, tuning=tuning)
+            e1.record()
+            torch.cuda.synchronize()
+            total_ms += e0.elapsed_time(e1)
             if r == rounds - 1:
-                e1.record()
-                torch.cuda.synchronize()
-                total_ms += e0.elapsed_time(e1)
                 alloc.write_pattern(offs, sizes, salt=7)
                 res = alloc.verify_pattern(offs, sizes, salt=7)
                 ok = ok and res["mismatches"] == 0 and res["misaligned"] == 0 and res["outside"] == 0
                 nulls += res["nulls"]
-                e0.record()
+            e0.record()
             alloc.wave_free_batch(offs, tuning=tuning)
             e1.record()
             torch.cuda.synchronize()
-            if r == rounds - 1:
-                total_ms += e0.elapsed_time(e1)
-                n_ops += 2 * sizes.numel()
-    return n_ops / (total_ms / 1e3), nulls, ok
+            total_ms += e0.elapsed_time(e1)
+            n_ops += 2 * sizes.numel()
+        if r >= rounds - timed:
+            rates.append(n_ops / (total_ms / 1e3))
+    return statistics.median(rates), nulls, ok
 
 
-def tune(base: mmc.ScatterConfig, heap_bytes: int, size_batches, space=None, log=None, max_nulls: int = 0):
+def _same_point(a: TunePoint, b: TunePoint) -> bool:
+    return (a.pagesize, a.regionsize, a.aggregation, bool(a.persistent), a.xcd_slices) == (
+        b.pagesize, b.regionsize, b.aggregation, bool(b.persistent), b.xcd_slices)
+
+
+def tune(base: mmc.ScatterConfig, heap_bytes: int, size_batches, space=None, log=None, max_nulls: int = 0,
+         prefer: TunePoint | None = None, tie: float = 0.03, refine: int = 4, refine_passes: int = 5):
     """Returns (best TunePoint, all points).  One heap per geometry serves all
     wave tunings of that geometry (free-all restores it between points when
-    resetfreedpages is on; the restore is checked)."""
+    resetfreedpages is on; the restore is checked).  Points within `tie` of
+    the fastest valid point are taken as equal, since run-to-run noise is of
+    that order; among them `prefer`, when given and measured, wins, so that
+    repeated runs settle on one point.  The `refine` fastest points of the
+    sweep (and `prefer`) are measured again over `refine_passes` timed passes
+    each before that choice."""
     space = space or DEFAULT_SPACE
     points = []
     for pagesize, regionsize in itertools.product(space["pagesize"], space["regionsize"]):
@@ -108,5 +125,25 @@ def tune(base: mmc.ScatterConfig, heap_bytes: int, size_batches, space=None, log
     valid = [p for p in points if p.valid]
     if not valid:
         raise mmc.MallocMCError("no valid tuning point")
-    best = max(valid, key=lambda p: p.ops_per_s)
+    # one timed pass per point is noisy: re-measure the leaders (and the
+    # preferred point) over `refine_passes` timed passes and rank by the median
+    leaders = sorted(valid, key=lambda p: p.ops_per_s, reverse=True)[:refine]
+    if prefer is not None:
+        leaders += [p for p in valid if _same_point(p, prefer) and p not in leaders]
+    for p in leaders:
+        cfg = dataclasses.replace(base, pagesize=p.pagesize, regionsize=p.regionsize)
+        alloc = mmc.ScatterAllocator(cfg, heap_bytes)
+        try:
+            ops, nulls, ok = measure(alloc, size_batches, p.tuning(), rounds=refine_passes + 1, timed=refine_passes)
+        finally:
+            alloc.close()
+        p.ops_per_s, p.valid = ops, p.valid and ok and nulls <= max_nulls
+        if log:
+            log(f"tune refine {p}")
+    leaders = [p for p in leaders if p.valid] or valid
+    best = max(leaders, key=lambda p: p.ops_per_s)
+    if prefer is not None:
+        for p in leaders:
+            if _same_point(p, prefer) and p.ops_per_s >= (1.0 - tie) * best.ops_per_s:
+                return p, points
     return best, points
