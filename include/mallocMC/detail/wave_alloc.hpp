@@ -15,6 +15,10 @@
  *   * a page serves the whole group with one atomicAdd(count, k) reservation
  *     and one vector atomicOr per touched 32-chunk mask word; chunk indices
  *     are handed to lanes by ballot rank;
+ *   * pages the scan saw unformatted, and flat pages (chunks in the PTE
+ *     bitmask), are claimed several at a time with the format, reservation
+ *     and bit claim in one round trip; when other waves took the scanned
+ *     pages meanwhile, the window is read again;
  *   * frees are grouped by page: one atomicAnd per touched mask word, one
  *     atomicAnd on the segment hints and one atomicSub(count, k).
  *
@@ -41,6 +45,10 @@ namespace mallocMC
 {
     namespace detail
     {
+        //! Reads of one scan window a group may make when its claims find the
+        //! window's snapshot stale (pages taken by other waves meanwhile).
+        constexpr u32 kWindowReads = 3u;
+
         //! Tuning knobs of the wave path (autotuned by the batched runner).
         struct WaveTuning
         {
@@ -52,12 +60,19 @@ namespace mallocMC
             //! Size classes per octave = 2^classSplit (0..3): finer classes waste
             //! less (chunk <= (1 + 2^-classSplit) x request) but aggregate fewer lanes.
             u32 classSplit = 0u;
-            //! Claim chunks above the hierarchy threshold on several pages in
-            //! one round trip (claimFlat).
+            //! Claim on several pages per round trip: flat pages from their
+            //! PTE bitmask (claimFlat), unformatted hierarchical pages from
+            //! their first segment words (claimFresh, unless freshFast is 0);
+            //! 0 claims every page on its own (claimOnPage).
             u32 flatFast = 1u;
             //! After a window without any candidate page, skip ahead 256 pages
             //! per round trip (fastForward).
             u32 fastForward = 1u;
+            //! With flatFast: claim unformatted hierarchical pages several per
+            //! round trip (claimFresh); 0 formats them one at a time through
+            //! claimOnPage.  Many small groups of one size class (the
+            //! multi-heap step's 64-slot tiles) can run faster without.
+            u32 freshFast = 1u;
         };
 
         //! Wave-wide reductions and scans.  All 64 lanes must be active.
@@ -445,9 +460,10 @@ namespace mallocMC
             //! claimBits, so picking from a stale bitmask only costs chunks,
             //! never hands one out twice.  A page that turned out locked by a
             //! reset, or whose reservation saw another chunk size, gives back
-            //! what it won.  Returns the lanes served.
+            //! what it won.  Returns the lanes served; `failed` gets the chosen
+            //! pages that yielded no chunk (the scan's view of them was stale).
             MALLOCMC_D u64 claimFlat(u32 pageL, u32 pcs, u32 pcount, u32 pbits, u64 used, u64 fresh, u32 rot,
-                                     u32 c, u64 need, u64& chosen, char*& out) const
+                                     u32 c, u64 need, u64& chosen, u64& failed, char*& out) const
             {
                 const u32 me = hw::laneid();
                 const u32 P = h.pagesize;
@@ -529,6 +545,7 @@ namespace mallocMC
                     else if(kept < take)
                         atom::sub(h.pteCount(pageL), take - kept); // holds kept chunks: count stays > 0
                 }
+                failed = hw::ballot(take != 0u && kept == 0u);
                 // hand the kept chunks to the group's lanes in rank order
                 const u32 myRank = hw::rankBelow(need);
                 const bool inNeed = ((need >> me) & 1u) != 0u;
@@ -543,6 +560,117 @@ namespace mallocMC
                         const u32 pg = wave::lane(pageL, l);
                         const u32 cs = wave::lane(csl, l);
                         out = h.pageData(pg) + static_cast<u64>(wave::selectBit(bits, myRank - got)) * cs;
+                    }
+                    got += n;
+                }
+                return hw::ballot(inNeed && myRank < got);
+            }
+
+            //! Claims chunks of a hierarchical chunk size c for the lanes of
+            //! `need` on several pages the scan saw unformatted, in one round
+            //! trip.  Lane l stands for page `pageL` (candidates in `fresh`);
+            //! pages are taken from the rotation point on until their capacity
+            //! covers the group.  Each chosen page's lane formats its page (CAS
+            //! 0 -> c), reserves `take` slots with the 64-bit add (which returns
+            //! the chunk size it was made under) and sets the lowest `take`
+            //! chunk bits in the page's first two segment words, all at once: a
+            //! group takes at most 64 chunks, and every hierarchical chunk size
+            //! has at least one full 32-chunk segment, so two words suffice.
+            //! The reference's tryUsePage (Scatter.hpp:469-517) reserves, then
+            //! re-checks the chunk size, then claims a bit, one round trip each;
+            //! claimOnPage needs two.  Ownership of a chunk is decided by the
+            //! atomicOr's result, as in claimBits: a concurrent claim of the same
+            //! page only costs chunks.  A page that turned out formatted with
+            //! another size, or locked by a reset, gives back what it won.
+            //! Returns the lanes served; `failed` gets the chosen pages that
+            //! yielded no chunk.
+            MALLOCMC_D u64 claimFresh(u32 pageL, u64 fresh, u32 rot, u32 c, u64 need, u64& chosen, u64& failed,
+                                      char*& out) const
+            {
+                const u32 me = hw::laneid();
+                const u32 P = h.pagesize;
+                const bool isFresh = ((fresh >> me) & 1u) != 0u;
+                const u32 cap = h.pageCapacity(c);
+                // order among the candidates: from the rotation point, then the rest
+                const u64 hiMask = rot < 64u ? (~0ull << rot) : 0ull;
+                const u64 below = hw::lanemask_lt();
+                const u32 nHi = static_cast<u32>(hw::popc(fresh & hiMask));
+                const bool inHi = isFresh && ((hiMask >> me) & 1u) != 0u;
+                const u32 order = inHi ? static_cast<u32>(hw::popc(fresh & hiMask & below))
+                                       : nHi + static_cast<u32>(hw::popc(fresh & ~hiMask & below));
+                const u32 k = static_cast<u32>(hw::popc(need));
+                const u32 pos = order * cap;
+                const u32 take = (isFresh && pos < k) ? umin32(cap, k - pos) : 0u;
+                const Layout L = layout(pageL, c);
+                const u32 sp0 = L.full >= 1u ? 32u : L.additional;
+                const u32 sp1 = L.full >= 2u ? 32u : (L.full == 1u ? L.additional : 0u);
+                const u32 t0 = umin32(take, sp0);
+                const u32 t1 = umin32(take - t0, sp1);
+                const u32 m0 = t0 >= 32u ? ~0u : ((1u << t0) - 1u);
+                const u32 m1 = t1 >= 32u ? ~0u : ((1u << t1) - 1u);
+                chosen = hw::ballot(take != 0u);
+                u32 old = 0u, rcs = 0u, fcs = 1u, ob0 = 0u, ob1 = 0u;
+                if(take != 0u)
+                {
+                    fcs = atom::cas(h.pteChunksize(pageL), 0u, c);
+                    const u64 pr = atom::add(h.ptePair(pageL), u64{take} << 32);
+                    ob0 = atom::or_(&L.words[0], m0);
+                    if(m1 != 0u)
+                        ob1 = atom::or_(&L.words[1], m1);
+                    old = static_cast<u32>(pr >> 32);
+                    rcs = static_cast<u32>(pr);
+                }
+                const bool formatted = take != 0u && fcs == 0u;
+                const bool locked = h.resetfreedpages && old >= P;
+                const bool usable = take != 0u && rcs == c && !locked;
+                const u32 won0 = take != 0u ? (m0 & ~ob0) : 0u;
+                const u32 won1 = take != 0u ? (m1 & ~ob1) : 0u;
+                const u32 keep0 = usable ? won0 : 0u;
+                const u32 keep1 = usable ? won1 : 0u;
+                const u32 kept = static_cast<u32>(hw::popc(keep0) + hw::popc(keep1));
+                if(take != 0u)
+                {
+                    if((won0 & ~keep0) != 0u)
+                        atom::and_(&L.words[0], ~(won0 & ~keep0));
+                    if((won1 & ~keep1) != 0u)
+                        atom::and_(&L.words[1], ~(won1 & ~keep1));
+                    if(kept == 0u)
+                    {
+                        backOff(pageL, take);
+                        if(formatted && locked)
+                        {
+                            // formatted a page whose reset still held the count
+                            // lock: once the reset is done, take the chunk size
+                            // back unless someone uses the page (D5)
+                            u32 spins = 0u;
+                            while(atom::load(h.pteCount(pageL)) >= P && spins < kResetWaitLimit)
+                            {
+                                __builtin_amdgcn_s_sleep(1);
+                                ++spins;
+                            }
+                            resetEmpty(pageL);
+                        }
+                    }
+                    else if(kept < take)
+                        atom::sub(h.pteCount(pageL), take - kept); // holds kept chunks: count stays > 0
+                }
+                failed = hw::ballot(take != 0u && kept == 0u);
+                // hand the kept chunks to the group's lanes in rank order
+                const u32 myRank = hw::rankBelow(need);
+                const bool inNeed = ((need >> me) & 1u) != 0u;
+                u32 got = 0u;
+                for(u64 pagesKept = hw::ballot(kept != 0u); pagesKept != 0u; pagesKept &= pagesKept - 1u)
+                {
+                    const u32 l = hw::ffs(pagesKept) - 1u;
+                    const u32 b0 = wave::lane(keep0, l);
+                    const u32 b1 = wave::lane(keep1, l);
+                    const u32 n0 = static_cast<u32>(hw::popc(b0));
+                    const u32 n = n0 + static_cast<u32>(hw::popc(b1));
+                    if(inNeed && myRank >= got && myRank < got + n)
+                    {
+                        const u32 r = myRank - got;
+                        const u32 idx = r < n0 ? wave::selectBit(b0, r) : 32u + wave::selectBit(b1, r - n0);
+                        out = h.pageData(wave::lane(pageL, l)) + static_cast<u64>(idx) * c;
                     }
                     got += n;
                 }
@@ -663,8 +791,13 @@ namespace mallocMC
                 const u32 R = h.regionsize;
                 const u32 ppb = h.pagesperblock;
                 const u32 numregions = h.numregions;
+                // the wave term of Scatter.hpp:535-538; the wave's id within its CU
+                // (slot and SIMD) rather than the slot alone, which the same-slot
+                // waves of the CU's four SIMDs share: with it, the waves of a
+                // 1,024-thread block that serve one size class started their
+                // walks on the same page and raced for the same pages
                 const u32 hashBase = c * h.hashingK + h.hashingDistMP * hw::cuid()
-                    + (h.hashingDistWP + h.hashingDistWPRel * h.divP.div(64u * c)) * hw::waveslot();
+                    + (h.hashingDistWP + h.hashingDistWPRel * h.divP.div(64u * c)) * hw::waveInCu();
                 u32 startInBlock;
                 if(t.xcdSlices > 1u && ppb >= t.xcdSlices * R)
                 {
@@ -694,7 +827,10 @@ namespace mallocMC
                 // chunks above the hierarchy threshold: every fitting page keeps
                 // its chunk bits in the PTE bitmask (candidates have a chunk
                 // size >= bmax = c)
-                const bool flat = c > h.hierarchyThreshold && t.flatFast != 0u;
+                const bool flat = c > h.hierarchyThreshold;
+                // claims on several pages per round trip (claimFlat, claimFresh);
+                // with flat_single every page is claimed on its own (claimOnPage)
+                const bool multi = t.flatFast != 0u;
                 for(u32 finder = 0; finder < 2u && need != 0u; ++finder)
                 {
                     for(u32 done = 0; done < numregions && need != 0u; done += W)
@@ -704,83 +840,144 @@ namespace mallocMC
                         // lane -> (region of this round, page within region)
                         const u32 ri = R >= 64u ? 0u : h.divR.div(me);
                         const u32 region = wrap(region0 + done + ri);
-                        u32 fill = 0xffffffffu, pcs = 0u, pcount = 0u, pbits = 0u;
                         const bool inSpan = me < span && ri < nreg;
                         for(u32 base = 0; base < (R >= 64u ? R : 1u) && need != 0u; base += 64u)
                         {
-                            const u32 pj = R >= 64u ? base + me : me - ri * R;
-                            const bool valid = inSpan && pj < R;
-                            if(valid)
+                            // Claims race with the other waves that scanned the same
+                            // pages: a round whose pages were taken meanwhile stops and
+                            // the window is read again, instead of trying the rest of a
+                            // stale snapshot page by page (at most kWindowReads reads).
+                            u64 raised = 0u; // regions of the window whose fill level this visit raised
+                            for(u32 read = 0; read < kWindowReads && need != 0u; ++read)
                             {
-                                fill = atom::load(&h.regions[region]);
-                                const u64 pr = atom::load(h.ptePair(region * R + pj));
-                                pcs = static_cast<u32>(pr);
-                                pcount = static_cast<u32>(pr >> 32);
-                                if(flat)
-                                    pbits = atom::load(h.pteBitmask(region * R + pj));
-                            }
-                            const bool regionOpen = valid && fill < checklevel;
-                            bool cand = false;
-                            if(regionOpen)
-                            {
-                                if(pcs == 0u)
-                                    cand = true;
-                                else if(pcs >= bmax && pcs <= maxcs && pcs <= h.pagesize)
-                                    cand = pcount < h.pageCapacity(pcs);
-                            }
-                            const u64 usedAll = hw::ballot(cand && pcs != 0u);
-                            const u64 freeAll = hw::ballot(cand && pcs == 0u);
-                            const u64 open = hw::ballot(regionOpen);
-                            windowHadCand = windowHadCand || (usedAll | freeAll) != 0u;
-                            const u32 rounds = R >= 64u ? 1u : nreg;
-                            for(u32 k = 0; k < rounds && need != 0u; ++k)
-                            {
-                                const u32 lo = R >= 64u ? 0u : k * R;
-                                const u32 width = R >= 64u ? umin32(64u, R - base) : R;
-                                const u64 sel = (width >= 64u ? ~0ull : ((1ull << width) - 1u)) << lo;
-                                if((open & sel) == 0u)
-                                    continue; // region filled past the threshold
-                                const u32 fillK = wave::lane(fill, lo);
-                                const u32 regionK = R >= 64u ? region : wrap(region0 + done + k);
-                                u64 used = usedAll & sel;
-                                u64 cands = freeAll & sel;
-                                const u32 rotAbs = R >= 64u ? (off0 >= base ? off0 - base : 0u) : off0;
-                                const u32 rot = lo + rotAbs;
-                                const u64 before = need;
-                                if(flat)
+                                bool stale = false;
+                                u32 fill = 0xffffffffu, pcs = 0u, pcount = 0u, pbits = 0u;
+                                const u32 pj = R >= 64u ? base + me : me - ri * R;
+                                const bool valid = inSpan && pj < R;
+                                if(valid)
                                 {
-                                    // two multi-page rounds over the scanned candidates; pages a
-                                    // round chose are not offered again
-                                    const u32 pageL = regionK * R + (R >= 64u ? base + me : me - lo);
-                                    for(u32 fr = 0; fr < 2u && need != 0u && (used | cands) != 0u; ++fr)
+                                    fill = atom::load(&h.regions[region]);
+                                    const u64 pr = atom::load(h.ptePair(region * R + pj));
+                                    pcs = static_cast<u32>(pr);
+                                    pcount = static_cast<u32>(pr >> 32);
+                                    if(flat && multi)
+                                        pbits = atom::load(h.pteBitmask(region * R + pj));
+                                }
+                                const bool regionOpen = valid && fill < checklevel;
+                                bool cand = false;
+                                if(regionOpen)
+                                {
+                                    if(pcs == 0u)
+                                        cand = true;
+                                    else if(pcs >= bmax && pcs <= maxcs && pcs <= h.pagesize)
+                                        cand = pcount < h.pageCapacity(pcs);
+                                }
+                                const u64 usedAll = hw::ballot(cand && pcs != 0u);
+                                const u64 freeAll = hw::ballot(cand && pcs == 0u);
+                                const u64 open = hw::ballot(regionOpen);
+                                windowHadCand = windowHadCand || (usedAll | freeAll) != 0u;
+                                const u32 rounds = R >= 64u ? 1u : nreg;
+                                for(u32 k = 0; k < rounds && need != 0u; ++k)
+                                {
+                                    const u32 lo = R >= 64u ? 0u : k * R;
+                                    const u32 width = R >= 64u ? umin32(64u, R - base) : R;
+                                    const u64 sel = (width >= 64u ? ~0ull : ((1ull << width) - 1u)) << lo;
+                                    if((open & sel) == 0u)
+                                        continue; // region filled past the threshold
+                                    const u32 fillK = wave::lane(fill, lo);
+                                    const u32 regionK = R >= 64u ? region : wrap(region0 + done + k);
+                                    u64 used = usedAll & sel;
+                                    u64 cands = freeAll & sel;
+                                    const u32 rotAbs = R >= 64u ? (off0 >= base ? off0 - base : 0u) : off0;
+                                    const u32 rot = lo + rotAbs;
+                                    const u64 before = need;
+                                    bool staleK = false;
+                                    if(multi)
                                     {
-                                        u64 chosen = 0u;
-                                        need &= ~claimFlat(pageL, pcs, pcount, pbits, used, cands, rot, c, need, chosen,
-                                                           out);
-                                        used &= ~chosen;
-                                        cands &= ~chosen;
+                                        const u32 pageL = regionK * R + (R >= 64u ? base + me : me - lo);
+                                        if(flat)
+                                        {
+                                            // rounds over the scanned candidates, used pages
+                                            // first; pages a round chose are not offered again
+                                            while(need != 0u && (used | cands) != 0u)
+                                            {
+                                                u64 chosen = 0u, failed = 0u;
+                                                need &= ~claimFlat(pageL, pcs, pcount, pbits, used, cands, rot, c,
+                                                                   need, chosen, failed, out);
+                                                used &= ~chosen;
+                                                cands &= ~chosen;
+                                                staleK = staleK || failed != 0u;
+                                                if(chosen == 0u || failed != 0u)
+                                                    break;
+                                            }
+                                        }
+                                        else
+                                        {
+                                            // partially used pages of a fitting chunk size first,
+                                            // one claim each: formatting a fresh page while a
+                                            // fitting one has room is what fragments the
+                                            // reference's heap; then the unformatted pages,
+                                            // several per round trip
+                                            while(used != 0u && need != 0u)
+                                            {
+                                                const u64 hi = rot < 64u ? (used >> rot) << rot : 0u;
+                                                const u32 l = hw::ffs(hi != 0u ? hi : used) - 1u;
+                                                used &= ~(1ull << l);
+                                                const u32 page = regionK * R + (R >= 64u ? base + l : l - lo);
+                                                const u64 served = claimOnPage(page, wave::lane(pcs, l), bmax, maxcs,
+                                                                               need, false, out);
+                                                need &= ~served;
+                                                if(served == 0u)
+                                                {
+                                                    staleK = true;
+                                                    break;
+                                                }
+                                            }
+                                            while(t.freshFast != 0u && need != 0u && cands != 0u && !staleK)
+                                            {
+                                                u64 chosen = 0u, failed = 0u;
+                                                need &= ~claimFresh(pageL, cands, rot, c, need, chosen, failed, out);
+                                                cands &= ~chosen;
+                                                staleK = staleK || failed != 0u;
+                                                if(chosen == 0u || failed != 0u)
+                                                    break;
+                                            }
+                                        }
+                                        used = 0u;
+                                        if(flat || t.freshFast != 0u)
+                                            cands = 0u;
+                                    }
+                                    // one page at a time (flat_single, and the unformatted pages
+                                    // of fresh_single), partially used pages of a fitting chunk
+                                    // size first, free pages after them
+                                    while((used | cands) != 0u && need != 0u)
+                                    {
+                                        const u64 set = used != 0u ? used : cands;
+                                        const u64 hi = rot < 64u ? (set >> rot) << rot : 0u;
+                                        const u64 pick = hi != 0u ? hi : set;
+                                        const u32 l = hw::ffs(pick) - 1u;
+                                        used &= ~(1ull << l);
+                                        cands &= ~(1ull << l);
+                                        const u32 page = regionK * R + (R >= 64u ? base + l : l - lo);
+                                        const u32 pc = wave::lane(pcs, l);
+                                        // a free page is formatted by the claim itself
+                                        const u64 served = claimOnPage(page, pc == 0u ? c : pc, bmax, maxcs, need,
+                                                                       pc == 0u, out);
+                                        need &= ~served;
+                                    }
+                                    stale = stale || staleK;
+                                    // an unsuccessful visit raises the region's fill level
+                                    // once (Scatter.hpp:612-617); a visit cut short by a
+                                    // stale snapshot is not one yet
+                                    if(need == before && !staleK)
+                                    {
+                                        if(((raised >> k) & 1u) == 0u && fillK + 1u <= R && me == 0u)
+                                            atom::cas(&h.regions[regionK], fillK, fillK + 1u);
+                                        raised |= 1ull << k;
                                     }
                                 }
-                                // partially used pages of a fitting chunk size first, free
-                                // pages after them: formatting a fresh page while a fitting
-                                // one has room is what fragments the reference's heap
-                                while((used | cands) != 0u && need != 0u)
-                                {
-                                    const u64 set = used != 0u ? used : cands;
-                                    const u64 hi = rot < 64u ? (set >> rot) << rot : 0u;
-                                    const u64 pick = hi != 0u ? hi : set;
-                                    const u32 l = hw::ffs(pick) - 1u;
-                                    used &= ~(1ull << l);
-                                    cands &= ~(1ull << l);
-                                    const u32 page = regionK * R + (R >= 64u ? base + l : l - lo);
-                                    const u32 pc = wave::lane(pcs, l);
-                                    // a free page is formatted by the claim itself
-                                    const u64 served = claimOnPage(page, pc == 0u ? c : pc, bmax, maxcs, need,
-                                                                   pc == 0u, out);
-                                    need &= ~served;
-                                }
-                                if(need == before && fillK + 1u <= R && me == 0u)
-                                    atom::cas(&h.regions[regionK], fillK, fillK + 1u);
+                                if(!stale)
+                                    break;
                             }
                         }
                         // access-block cursor: advanced when the walk enters a new block

@@ -110,6 +110,14 @@ namespace mallocMC
             return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 4);
         }
 
+        //! Wave slot and SIMD within the CU (HW_ID[5:0]): unlike waveslot(),
+        //! distinct for waves of one CU that sit in the same slot of
+        //! different SIMDs.
+        MALLOCMC_D std::uint32_t waveInCu()
+        {
+            return __builtin_amdgcn_s_getreg((5 << 11) | (0 << 6) | 4);
+        }
+
         //! Compute unit id including SE and XCD bits (HIP's __smid on gfx950).
         MALLOCMC_D std::uint32_t cuid()
         {

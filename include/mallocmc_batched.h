@@ -24,8 +24,8 @@ typedef struct mmc_tuning {
     uint32_t persistent;   /* 0: one lane per request; 1: persistent waves pull 64-request tiles */
     uint32_t waves_per_cu; /* persistent launch size (0 = 8) */
     uint32_t class_split;  /* size classes per octave = 2^class_split (0..3) */
-    uint32_t flat_single;  /* 1: claim pages of chunks above the hierarchy threshold one at a time
-                              (0, the default: several pages per round trip) */
+    uint32_t flat_single;  /* 1: claim every page on its own, two round trips each (0, the default:
+                              flat pages and unformatted pages several per round trip) */
     uint32_t no_skip;      /* 1: walk every window of the region scan (0, the default: after a window
                               without candidate pages, skip ahead 256 pages per round trip) */
     uint32_t no_partition; /* 1: lane i of a non-persistent launch serves request i (0, the default:
@@ -34,6 +34,9 @@ typedef struct mmc_tuning {
     uint32_t block_threads; /* threads per block of the non-persistent wave malloc and free, of the
                                non-persistent multi-heap step and of the fill, the requests one
                                partition sorts: 256, 512 or 1024 (0 = 1024) */
+    uint32_t fresh_single;  /* 1: claim the unformatted pages of chunk sizes with a segment hierarchy
+                               one at a time, two round trips each (0, the default: several per round
+                               trip; flat_single = 1 implies this) */
 } mmc_tuning;
 
 int mmc_default_tuning(mmc_tuning* t);

@@ -59,6 +59,7 @@ class MmcTuning(ctypes.Structure):
         ("no_skip", ctypes.c_uint32),
         ("no_partition", ctypes.c_uint32),
         ("block_threads", ctypes.c_uint32),
+        ("fresh_single", ctypes.c_uint32),
     ]
 
 
@@ -66,12 +67,14 @@ class MmcTuning(ctypes.Structure):
 class Tuning:
     """Wave-path tuning (mmc_tuning): aggregation width 1..64, XCD slices,
     persistent launch and its waves per CU, size classes per octave,
-    whether flat pages are claimed one at a time (flat_single), whether
+    whether pages are claimed one at a time (flat_single), whether
     the region scan walks every window (no_skip) and whether a block serves
     its requests in arrival order instead of partitioned by size
-    (no_partition), and the threads per block of the non-persistent wave
+    (no_partition), the threads per block of the non-persistent wave
     malloc and free, of the non-persistent multi-heap step and of the fill
-    (block_threads: 256, 512 or 1024; 0 = 1024)."""
+    (block_threads: 256, 512 or 1024; 0 = 1024), and whether unformatted
+    pages of hierarchical chunk sizes are claimed one at a time
+    (fresh_single)."""
     aggregation: int = 64
     xcd_slices: int = 0
     persistent: bool = False
@@ -81,11 +84,12 @@ class Tuning:
     no_skip: bool = False
     no_partition: bool = False
     block_threads: int = 0
+    fresh_single: bool = False
 
     def to_c(self) -> MmcTuning:
         return MmcTuning(self.aggregation, self.xcd_slices, int(bool(self.persistent)), self.waves_per_cu,
                          self.class_split, int(bool(self.flat_single)), int(bool(self.no_skip)),
-                         int(bool(self.no_partition)), self.block_threads)
+                         int(bool(self.no_partition)), self.block_threads, int(bool(self.fresh_single)))
 
 
 class MmcStepDesc(ctypes.Structure):

@@ -49,6 +49,7 @@ namespace
             w.classSplit = t->class_split > 3u ? 3u : t->class_split;
             w.flatFast = t->flat_single ? 0u : 1u;
             w.fastForward = t->no_skip ? 0u : 1u;
+            w.freshFast = t->fresh_single ? 0u : 1u;
         }
         return w;
     }
@@ -734,6 +735,7 @@ int mmc_default_tuning(mmc_tuning* t)
     t->no_skip = 0u;
     t->no_partition = 0u;
     t->block_threads = 0u;
+    t->fresh_single = 0u;
     return MMC_OK;
 }
 

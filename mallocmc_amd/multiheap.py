@@ -147,23 +147,26 @@ LAUNCH_FORMS = ((True, 0), (False, 256), (False, 512), (False, 1024))
 
 def _tuning(p: dict) -> mmc.Tuning:
     return mmc.Tuning(aggregation=p["aggregation"], persistent=p["persistent"], block_threads=p["block_threads"],
-                      xcd_slices=p["xcd_slices"])
+                      xcd_slices=p["xcd_slices"], fresh_single=p.get("fresh_single", False))
 
 
 def run_default(log=None, steps: int = 100, tune_steps: int = 20, space=None) -> dict:
     """Autotunes in two stages (pagesize x regionsize x launch form at the
-    widest aggregation and 8 XCD slices, then aggregation x XCD slicing at
+    widest aggregation, 8 XCD slices and one-at-a-time claims of unformatted
+    pages (fresh_single), then aggregation x XCD slicing x fresh_single at
     the best point), then runs `steps` graph-replayed steps at the best point
-    and the reference geometry."""
+    and the reference geometry.  The first stage takes fresh_single because
+    the multi-page claims of unformatted pages, faster for one large batch,
+    slow some launch forms of this step down by a quarter."""
     work = make_workload()
     space = space or dict(pagesize=(4096, 16384, 65536), regionsize=(8, 16, 32), aggregation=(1, 16, 64),
-                          launch=LAUNCH_FORMS, xcd_slices=(8, 0))
+                          launch=LAUNCH_FORMS, xcd_slices=(8, 0), fresh_single=(True, False))
     points = []
     t0 = time.time()
 
-    def measure(P, R, agg, persistent, block_threads, xcd):
+    def measure(P, R, agg, persistent, block_threads, xcd, fresh_single=False):
         point = dict(pagesize=P, regionsize=R, aggregation=agg, persistent=persistent, block_threads=block_threads,
-                     xcd_slices=xcd)
+                     xcd_slices=xcd, fresh_single=fresh_single)
         run = MultiRun(work, P, R)
         tun = _tuning(point)
         run.run(3, tun)  # warm up into the steady state
@@ -176,13 +179,14 @@ def run_default(log=None, steps: int = 100, tune_steps: int = 20, space=None) ->
 
     widest = max(space["aggregation"])
     xcds = space.get("xcd_slices", (8,))
+    fresh = space.get("fresh_single", (False,))
     for P, R in itertools.product(space["pagesize"], space["regionsize"]):
         for persistent, bt in space["launch"]:
-            measure(P, R, widest, persistent, bt, xcds[0])
+            measure(P, R, widest, persistent, bt, xcds[0], fresh[0])
     b = max(points, key=lambda p: p["ops_per_s"])
-    for agg, xcd in itertools.product(space["aggregation"], xcds):
-        if (agg, xcd) != (widest, xcds[0]):
-            measure(b["pagesize"], b["regionsize"], agg, b["persistent"], b["block_threads"], xcd)
+    for agg, xcd, fs in itertools.product(space["aggregation"], xcds, fresh):
+        if (agg, xcd, fs) != (widest, xcds[0], fresh[0]):
+            measure(b["pagesize"], b["regionsize"], agg, b["persistent"], b["block_threads"], xcd, fs)
     best = max(points, key=lambda p: p["ops_per_s"])
     tune_s = time.time() - t0
     run = MultiRun(work, best["pagesize"], best["regionsize"])

@@ -82,18 +82,21 @@ def test_snapshot_payload_holds_the_pattern(mmc, cuda):
     a.close()
 
 
-@pytest.mark.parametrize("persistent,block_threads", [(True, 0), (False, 512)])
-def test_snapshot_after_multi_heap_steps(mmc, cuda, persistent, block_threads):
+@pytest.mark.parametrize("persistent,block_threads,fresh_single", [(True, 0, False), (False, 512, False),
+                                                                   (False, 512, True)])
+def test_snapshot_after_multi_heap_steps(mmc, cuda, persistent, block_threads, fresh_single):
     """Concurrent mallocs and frees of the multi-heap step (persistent, or one
-    partitioning workgroup per 8 tiles) leave every heap consistent with the
-    slots' live allocations."""
+    partitioning workgroup per 8 tiles; unformatted pages claimed several per
+    round trip or one at a time) leave every heap consistent with the slots'
+    live allocations."""
     from mallocmc_amd import snapshot as S
     from test_gpu_wave import _step_setup
     torch = cuda
     nheaps, per = 4, 8192
     allocs, multi, dev, d, slot_heap = _step_setup(torch, mmc, nheaps, per, 128 * MiB, 4096, seed=13, nsizes=13)
-    tun = mmc.Tuning(persistent=persistent, aggregation=64, block_threads=block_threads)
-    print(f"snapshot after multi-heap steps persistent={persistent} block_threads={block_threads}", flush=True)
+    tun = mmc.Tuning(persistent=persistent, aggregation=64, block_threads=block_threads, fresh_single=fresh_single)
+    print(f"snapshot after multi-heap steps persistent={persistent} block_threads={block_threads} "
+          f"fresh_single={fresh_single}", flush=True)
     for _ in range(10):
         multi.step(d, tun)
     torch.cuda.synchronize()

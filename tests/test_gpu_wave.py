@@ -35,6 +35,7 @@ TUNINGS = [
     dict(aggregation=64, xcd_slices=0, persistent=False, no_partition=True),
     dict(aggregation=64, xcd_slices=8, persistent=False, block_threads=256),
     dict(aggregation=32, xcd_slices=0, persistent=False, block_threads=512),
+    dict(aggregation=64, xcd_slices=0, persistent=False, fresh_single=True),
 ]
 
 

@@ -36,6 +36,7 @@ class _FakeRun:
         ms += {0: 0.0, 256: 0.1, 512: 0.0, 1024: 0.2}[tun.block_threads]
         ms += 0.0 if tun.aggregation == 16 else 0.05
         ms += 0.0 if tun.xcd_slices == 0 else 0.02
+        ms += 0.0 if not tun.fresh_single else 0.03
         return ms * steps, np.array([1000 * steps, 1000 * steps, 0, 10 * steps])
 
     def check_invariants(self):
@@ -52,9 +53,10 @@ def test_two_stage_autotune_finds_the_optimum(monkeypatch):
     best = out["tuned"]
     assert (best["pagesize"], best["regionsize"]) == (65536, 32)
     assert (best["persistent"], best["block_threads"]) == (False, 512)
-    # the second stage at the best point finds the aggregation and XCD slicing
-    assert (best["aggregation"], best["xcd_slices"]) == (16, 0)
-    # 3 pagesizes x 3 regionsizes x 4 launch forms, then 3 x 2 - 1 second-stage points
-    assert out["autotune_points"] == 9 * len(mh.LAUNCH_FORMS) + 5
+    # the second stage at the best point finds the aggregation, XCD slicing
+    # and multi-page claims of unformatted pages
+    assert (best["aggregation"], best["xcd_slices"], best["fresh_single"]) == (16, 0, False)
+    # 3 pagesizes x 3 regionsizes x 4 launch forms, then 3 x 2 x 2 - 1 second-stage points
+    assert out["autotune_points"] == 9 * len(mh.LAUNCH_FORMS) + 11
     assert out["invariants_ok"] and _FakeRun.checked >= 1
     assert np.isclose(out["mallocs_frees_per_s"], 2000 / (1.0 / 1e3))
