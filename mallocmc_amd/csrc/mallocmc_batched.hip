@@ -475,6 +475,18 @@ namespace
         return cached[dev];
     }
 
+    //! Request size of lane `lane` in round `it` of a fill: the two-part
+    //! uniform mixture of mmc_fill_desc, from splitmix64(seed, lane, it).
+    __device__ u32 fillBytes(const mmc_fill_desc& d, u32 lane, u32 it)
+    {
+        const u64 r = mix64(d.seed ^ mix64((static_cast<u64>(lane) << 24) + it));
+        const u32 coin = static_cast<u32>(r % 1000u);
+        const u32 u = static_cast<u32>(r >> 32);
+        if(coin < d.p_small_permille)
+            return d.small_lo + u % (d.small_hi - d.small_lo + 1u);
+        return d.large_lo + u % (d.large_hi - d.large_lo + 1u);
+    }
+
     //! Fill until null (see mmc_fill_desc): each lane allocates until the heap
     //! returns null for it; the wave keeps calling the wave malloc until every
     //! lane stopped.
@@ -498,17 +510,7 @@ namespace
                 if(__builtin_amdgcn_readfirstlane(stop) != 0u)
                     break;
             }
-            u32 bytes = 0u;
-            if(active)
-            {
-                const u64 r = mix64(d.seed ^ mix64((static_cast<u64>(lane) << 24) + it));
-                const u32 coin = static_cast<u32>(r % 1000u);
-                const u32 u = static_cast<u32>(r >> 32);
-                if(coin < d.p_small_permille)
-                    bytes = d.small_lo + u % (d.small_hi - d.small_lo + 1u);
-                else
-                    bytes = d.large_lo + u % (d.large_hi - d.large_lo + 1u);
-            }
+            u32 bytes = active ? fillBytes(d, lane, it) : 0u;
             void* p = wa.malloc<NoopDistribution>(bytes);
             if(active)
             {
@@ -542,8 +544,13 @@ namespace
     //! rounds are workgroup-uniform (the partition has barriers): a
     //! workgroup runs while any of its lanes is active and no stop was seen.
     //! Each lane draws the same sizes as in fillKernel.  On the 64 GiB
-    //! stress fill, workgroups of 1024 lanes refill in 0.20-0.24 s, of 512 in
-    //! 0.27-0.30 s and of 256 in 0.44-0.48 s (arrival order: 1.0-1.2 s).
+    //! stress fill at regionsize 16, workgroups of 1024 lanes refilled in
+    //! 0.20-0.24 s, of 512 in 0.27-0.30 s and of 256 in 0.44-0.48 s (arrival
+    //! order: 1.0-1.2 s); at regionsize 32, 1024 lanes in 0.17-0.20 s.
+    //! A workgroup waits at every round for its longest walk (three quarters
+    //! of the waves' time on that fill), yet serving several rounds per
+    //! barrier, waves pulling sorted tiles, was 5-8x slower: the waves then
+    //! all walk at once and contend for the same last free pages.
     template<u32 B>
     __global__ __launch_bounds__(B) void fillPartitionKernel(ScatterHeap h, WaveTuning t, mmc_fill_desc d)
     {
@@ -559,17 +566,7 @@ namespace
             sStopped[threadIdx.x] = 0u;
             if(__syncthreads_or(active ? 1 : 0) == 0 || sStop != 0u)
                 break;
-            u32 bytes = 0u;
-            if(active)
-            {
-                const u64 r = mix64(d.seed ^ mix64((static_cast<u64>(lane) << 24) + it));
-                const u32 coin = static_cast<u32>(r % 1000u);
-                const u32 u = static_cast<u32>(r >> 32);
-                if(coin < d.p_small_permille)
-                    bytes = d.small_lo + u % (d.small_hi - d.small_lo + 1u);
-                else
-                    bytes = d.large_lo + u % (d.large_hi - d.large_lo + 1u);
-            }
+            u32 bytes = active ? fillBytes(d, lane, it) : 0u;
             u32 slot = threadIdx.x;
             partitionByKey<u32, B>(sizeBucket(bytes, h.pagesize), bytes, bytes, slot);
             void* p = wa.malloc<NoopDistribution>(bytes);
@@ -828,7 +825,8 @@ int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, c
 int mmc_wave_fill_until_null(mmc_heap* heap, const mmc_fill_desc* desc, const mmc_tuning* tuning, void* stream)
 {
     if(!heap || !desc || !desc->d_offsets || !desc->d_sizes || !desc->d_count || desc->lanes == 0
-       || desc->small_hi < desc->small_lo || desc->large_hi < desc->large_lo || desc->p_small_permille > 1000u)
+       || desc->small_hi < desc->small_lo || desc->large_hi < desc->large_lo || desc->p_small_permille > 1000u
+       || desc->small_hi - desc->small_lo == 0xffffffffu || desc->large_hi - desc->large_lo == 0xffffffffu)
         return mmc::fail(MMC_EINVAL, "bad fill descriptor");
     const WaveTuning t = toTuning(tuning);
     hipStream_t s = static_cast<hipStream_t>(stream);

@@ -4,11 +4,11 @@ used as the yardstick for the GPU stress cycles (tests only)."""
 import numpy as np
 
 
-def oracle_first_null_fill(oracle_mod, heap_bytes, cycles=3, pagesize=65536, seed=1):
+def oracle_first_null_fill(oracle_mod, heap_bytes, cycles=3, pagesize=65536, seed=1, regionsize=16):
     """Per cycle: allocate 80% U[16,256] B / 20% U[8 KiB, 60 KiB] requests
     until the first null, record payload bytes / page bytes and the free
     pages left, free a seeded random half.  Returns a list of dicts."""
-    cfg = oracle_mod.make_config(pagesize=pagesize, regionsize=16, accessblocksize=min(2 * 1024 ** 3, heap_bytes),
+    cfg = oracle_mod.make_config(pagesize=pagesize, regionsize=regionsize, accessblocksize=min(2 * 1024 ** 3, heap_bytes),
                                  resetfreedpages=1)
     out = []
     with oracle_mod.OracleHeap(cfg, heap_bytes) as h:

@@ -34,6 +34,29 @@ def test_oom_stress_cycles(mmc, cuda, split, first_null, block_threads, no_parti
             assert c["slots_after_fill"][3] == 0, c  # no free page at the first null
 
 
+def test_fill_rejects_bad_descriptors(mmc, cuda):
+    """A size range spanning all 32-bit values (its width would wrap to 0),
+    reversed ranges and permille above 1000 are refused before any launch."""
+    import ctypes
+    import torch
+    from mallocmc_amd import oomstress
+    L = oomstress._bind()
+    a = mmc.ScatterAllocator(mmc.ScatterConfig(pagesize=65536, regionsize=32), 64 * 1024 ** 2)
+    offs = torch.empty(16, dtype=torch.int64, device="cuda")
+    sizes = torch.empty(16, dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    good = dict(seed=1, lanes=64, p_small_permille=800, small_lo=16, small_hi=256, large_lo=8192, large_hi=61440,
+                cap=16, d_offsets=offs.data_ptr(), d_sizes=sizes.data_ptr(), d_count=count.data_ptr(), d_stop=None)
+    tun = mmc.Tuning().to_c()
+    for bad in (dict(small_lo=0, small_hi=0xFFFFFFFF), dict(large_lo=0, large_hi=0xFFFFFFFF),
+                dict(small_lo=300, small_hi=200), dict(p_small_permille=1001), dict(lanes=0)):
+        d = oomstress.MmcFillDesc(**{**good, **bad})
+        assert L.mmc_wave_fill_until_null(a.handle, ctypes.byref(d), ctypes.byref(tun), None) == -1, bad  # MMC_EINVAL
+    torch.cuda.synchronize()
+    assert int(count.item()) == 0
+    a.close()
+
+
 def test_oom_first_null_fill_close_to_reference_protocol(mmc, cuda, oracle_mod):
     """65,536 concurrent lanes on a 1 GiB heap reach, at their first null,
     a payload fill within 0.08 of what the serialized reference protocol
@@ -44,8 +67,8 @@ def test_oom_first_null_fill_close_to_reference_protocol(mmc, cuda, oracle_mod):
     from mallocmc_amd import oomstress
     from oom_fill import oracle_first_null_fill
     heap = 1024 ** 3
-    ref = oracle_first_null_fill(oracle_mod, heap, cycles=3)
-    r = oomstress.run_default(heap_bytes=heap, lanes=65536, cycles=3, log=print)
+    ref = oracle_first_null_fill(oracle_mod, heap, cycles=3, regionsize=32)
+    r = oomstress.run_default(heap_bytes=heap, lanes=65536, cycles=3, regionsize=32, log=print)
     assert r["invariants_ok"] and r["slots_restored"], r
     for c, o in zip(r["cycles"], ref):
         print(f"fill gpu {c['fill_fraction']:.3f} reference protocol {o['fill_fraction']:.3f}", flush=True)
