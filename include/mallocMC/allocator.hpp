@@ -3,9 +3,11 @@
  * Allocator<Acc, Creation, Distribution, OOM, ReservePool, Alignment>.
  * alpaka's device and queue become a HIP device ordinal and a hipStream_t.
  * Lifecycle as the reference's alloc() (:136-159): reserve the pool through
- * the ReservePool policy, align it with the Alignment policy, then build the
- * heap in it (C ABI mmc_create_in_pool, which enqueues the init kernel on the
- * queue).  Host errors throw, as the reference's do.
+ * the ReservePool policy, align it with the Alignment policy, then let the
+ * Creation policy build the heap in it (CreationPolicy::initHeap, which
+ * enqueues the init kernel on the queue and fills the device allocator).
+ * getAvailableSlots goes through CreationPolicy::getAvailableSlotsHost, as
+ * the reference's does (:60-72).  Host errors throw, as the reference's do.
  */
 #pragma once
 
@@ -104,13 +106,11 @@ namespace mallocMC
         //! getAvailableSlots (allocator.hpp:227-233): pads, counts, blocks on the queue.
         unsigned getAvailableSlots(int device, hipStream_t queue, std::size_t slotSize)
         {
-            (void) device;
             slotSize = AlignmentPolicy::applyPadding(static_cast<std::uint32_t>(slotSize));
             if constexpr(!Traits<Allocator>::providesAvailableSlots)
                 return 0u;
-            std::uint32_t out = 0;
-            check(mmc_available_slots(heap, slotSize, queue, &out), "getAvailableSlots");
-            return out;
+            else
+                return CreationPolicy::template getAvailableSlotsHost<T_Acc>(device, queue, slotSize, heap);
         }
 
         //! getHeapLocations (allocator.hpp:235-241)
@@ -139,47 +139,28 @@ namespace mallocMC
             }
         }
 
-        static mmc_config cConfig()
-        {
-            const detail::ScatterConfig r
-                = CreationPolicy::runtimeConfig(static_cast<std::uint32_t>(AlignmentPolicy::dataAlignment));
-            mmc_config c{};
-            c.pagesize = r.pagesize;
-            c.regionsize = r.regionsize;
-            c.accessblocksize = r.accessblocksize;
-            c.wastefactor = r.wastefactor;
-            c.resetfreedpages = r.resetfreedpages;
-            c.hashingK = r.hashingK;
-            c.hashingDistMP = r.hashingDistMP;
-            c.hashingDistWP = r.hashingDistWP;
-            c.hashingDistWPRel = r.hashingDistWPRel;
-            c.dataAlignment = r.dataAlignment;
-            c.distribution = DistributionPolicy::coalesces ? MMC_DIST_XMALLOCSIMD : MMC_DIST_NOOP;
-            c.flags = 0u;
-            return c;
-        }
-
         //! alloc (allocator.hpp:136-159)
         void alloc(int device, hipStream_t queue, std::size_t size)
         {
             if(hipSetDevice(device) != hipSuccess)
                 throw std::runtime_error("hipSetDevice failed");
-            void* pool = reservePolicy.setMemPool(device, size);
+            void* const raw = reservePolicy.setMemPool(device, size);
+            void* pool = raw;
             std::tie(pool, size) = AlignmentPolicy::alignPool(pool, size);
-            const mmc_config c = cConfig();
-            const int rc = mmc_create_in_pool(&c, pool, size, queue, &heap);
-            if(rc != MMC_OK)
+            try
             {
-                reservePolicy.resetMemPool(pool);
-                check(rc, "Allocator");
+                heap = CreationPolicy::template initHeap<T_Acc>(device, queue, &devAllocator, pool, size);
             }
-            std::uint64_t n = sizeof(detail::ScatterHeap);
-            check(mmc_get_handle(heap, &devAllocator.heap, &n), "getAllocatorHandle");
+            catch(...)
+            {
+                reservePolicy.resetMemPool(raw);
+                throw;
+            }
             devAllocator.pool = pool;
             void* p = nullptr;
             std::uint64_t s = 0;
             check(mmc_heap_location(heap, &p, &s), "getHeapLocations");
-            rawPool = pool;
+            rawPool = raw; // what the reserve policy handed out, before alignment
             heapInfos.p = p;
             heapInfos.size = static_cast<std::size_t>(s);
         }

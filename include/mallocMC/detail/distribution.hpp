@@ -17,10 +17,40 @@
 
 #include "scatter_heap.hpp"
 
+#include <type_traits>
+
 namespace mallocMC
 {
     namespace detail
     {
+        //! Whether a distribution policy serves several lanes from one
+        //! allocation (its `coalesces`; false for policies without one).
+        template<class P, class = void>
+        struct CoalescesOf : std::false_type
+        {
+        };
+        template<class P>
+        struct CoalescesOf<P, std::void_t<decltype(P::coalesces)>> : std::bool_constant<P::coalesces>
+        {
+        };
+        template<class P>
+        constexpr bool coalescesOf()
+        {
+            return CoalescesOf<P>::value;
+        }
+
+        //! Whether a distribution policy names the device type the
+        //! wave-cooperative path runs (its `Device`); a user-written policy
+        //! with only the concept members is served per lane.
+        template<class P, class = void>
+        struct HasWaveDevice : std::false_type
+        {
+        };
+        template<class P>
+        struct HasWaveDevice<P, std::void_t<typename P::Device>> : std::true_type
+        {
+        };
+
         struct NoopDistribution
         {
             template<class Heap>
@@ -42,12 +72,20 @@ namespace mallocMC
             u32 leader = 0;
             u32 reqSize = 0;
 
-            //! collect (XMallocSIMD.hpp:118-150); all lanes of the wave that
-            //! will call distribute() must call collect() together.
+            //! collect (XMallocSIMD.hpp:118-150) with the heap's page bound;
+            //! all lanes of the wave that will call distribute() must call
+            //! collect() together.
             template<class Heap>
             MALLOCMC_D u32 collect(const Heap& heap, u32 bytes)
             {
-                const bool coalescible = bytes > 0u && bytes < heap.pagesize / waveSize;
+                return collectBelow(heap.pagesize / waveSize, bytes);
+            }
+
+            //! collect with lanes of 0 < bytes < limit coalescible; limit =
+            //! pagesize / 64 keeps 16 + 64 * max(bytes) within a page.
+            MALLOCMC_D u32 collectBelow(u32 limit, u32 bytes)
+            {
+                const bool coalescible = bytes > 0u && bytes < limit;
                 const u64 m = hw::ballot(coalescible);
                 threadcount = hw::popc(m);
                 canCoalesce = coalescible && threadcount > 1u;

@@ -4,7 +4,8 @@
  * requests of a wavefront are served by one allocation with a 16-byte header
  * counting the slices.  On wave64 a lane joins when 0 < bytes < pagesize/64
  * (detail::XMallocSIMD64 explains why the reference's pagesize/32 bound
- * overflows a page with 64 lanes).
+ * overflows a page with 64 lanes).  collect / distribute take the
+ * accelerator, as the reference's members do (:90-176).
  */
 #pragma once
 
@@ -31,6 +32,7 @@ namespace mallocMC
         {
         public:
             using Properties = T_Config;
+            //! the same policy for the wave-cooperative path
             using Device = detail::XMallocSIMD64;
             static constexpr bool coalesces = true;
 #ifdef MALLOCMC_DP_XMALLOCSIMD_PAGESIZE
@@ -40,12 +42,42 @@ namespace mallocMC
 #endif
             static constexpr std::uint32_t _pagesize = pagesize;
 
+            MALLOCMC_HD XMallocSIMD() = default;
+
+            //! XMallocSIMD(acc) (XMallocSIMD.hpp:90-98): one object per call.
+            template<typename Acc>
+            MALLOCMC_D explicit XMallocSIMD(const Acc&)
+            {
+            }
+
+            //! collect (XMallocSIMD.hpp:118-150): a lane joins when
+            //! 0 < bytes < pagesize / 64; the first joining lane requests 16 +
+            //! the sum of the joining lanes' sizes, the others 0.  All lanes
+            //! of the wavefront that call distribute() call collect() together.
+            template<typename Acc>
+            MALLOCMC_D std::uint32_t collect(const Acc&, std::uint32_t bytes)
+            {
+                return state.collectBelow(pagesize / waveSize, bytes);
+            }
+
+            //! distribute (XMallocSIMD.hpp:152-176): the leader writes the
+            //! slice count into the 16-byte header; every joining lane gets
+            //! chunk + 16 + the sizes of the joining lanes below it.
+            template<typename Acc>
+            MALLOCMC_D void* distribute(const Acc&, void* allocatedMem)
+            {
+                return state.distribute(allocatedMem);
+            }
+
             static std::string classname()
             {
                 std::stringstream ss;
                 ss << "XMallocSIMD[" << pagesize << "]";
                 return ss.str();
             }
+
+        private:
+            detail::XMallocSIMD64 state;
         };
     } // namespace DistributionPolicies
 } // namespace mallocMC

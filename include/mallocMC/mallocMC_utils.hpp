@@ -25,6 +25,13 @@ namespace mallocMC
     //! Largest work-group the HIP runtime launches (used to size per-wave LDS arrays).
     constexpr std::uint32_t maxThreadsPerBlock = 1024u;
 
+    //! Accelerator tag of the gfx950 device: alpaka's accelerator object has
+    //! no counterpart in HIP device code, so the `acc` the policies take is a
+    //! tag; device functions accept any type.
+    struct AccGfx950
+    {
+    };
+
     template<class T>
     MALLOCMC_HD T divup(T a, T b)
     {

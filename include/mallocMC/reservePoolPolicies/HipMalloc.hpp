@@ -6,6 +6,8 @@
  */
 #pragma once
 
+#include "../mallocMC_utils.hpp"
+
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -14,11 +16,6 @@
 
 namespace mallocMC
 {
-    //! Accelerator tag of this implementation; device functions accept any acc.
-    struct AccGfx950
-    {
-    };
-
     namespace ReservePoolPolicies
     {
         struct HipMalloc

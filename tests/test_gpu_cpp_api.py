@@ -38,6 +38,15 @@ def test_verify_heap(heap_mb, blocks, threads):
     assert r.returncode == 0 and "verify_heap OK" in r.stdout
 
 
+def test_user_written_policies():
+    """A distribution policy and an OOM policy written against the
+    reference's policy concept (device_allocator.hpp:55-107) plug into the
+    Allocator: requests rounded to 64 B take one 64-byte slot each, every
+    null reaches the user's handleOOM on the per-lane and the wave path."""
+    r = _run("custom_policies")
+    assert r.returncode == 0 and "custom_policies OK" in r.stdout
+
+
 def test_thread_malloc_drop_in():
     """1M threads malloc and free through the drop-in handle: full
     wavefronts take the wave-cooperative path, wavefronts with a lane missing
