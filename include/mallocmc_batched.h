@@ -90,7 +90,9 @@ int mmc_multi_destroy(mmc_multi* m);
  * belongs to heap d_slot_heap[s]; slots of one heap are contiguous and each
  * 64-slot tile of d_tiles (uint32 first slot, tile i covers
  * [d_tiles[i], min(d_tiles[i] + 64, end of that heap's slots))) lies in one
- * heap.  Each step, for every slot: a live slot (d_offsets[s] != MMC_NULL)
+ * heap.  An entry MMC_NO_TILE is padding: it covers no slot, so a caller can
+ * start each heap's tiles at a multiple of the workgroup's block_threads/64
+ * and let every workgroup of the workgroup form partition its requests.  Each step, for every slot: a live slot (d_offsets[s] != MMC_NULL)
  * frees its allocation with probability p_free_permille/1000; a dead slot
  * allocates with probability p_alloc_permille/1000 a size drawn from the
  * table d_size_table (n_sizes entries) with the cumulative weights
@@ -98,6 +100,8 @@ int mmc_multi_destroy(mmc_multi* m);
  * (seed, *d_step, slot).  *d_step is incremented by the launch, so a captured
  * graph replays successive steps.  d_counters (uint64[4], accumulated):
  * mallocs, frees, nulls, tiles processed. */
+#define MMC_NO_TILE 0xffffffffu
+
 typedef struct mmc_step_desc {
     const uint32_t* d_slot_heap;
     const uint32_t* d_tiles;
@@ -120,8 +124,8 @@ typedef struct mmc_step_desc {
 /* One step.  tuning NULL or tuning->persistent = 1: persistent waves pull the
  * tiles from d_tile_cursor[0].  tuning->persistent = 0: one workgroup of
  * tuning->block_threads threads per block_threads/64 consecutive tiles; when
- * they lie in one heap the workgroup first reorders its frees by page and its
- * mallocs by size (unless tuning->no_partition).  The workgroup form
+ * its tiles with slots lie in one heap the workgroup first reorders its frees
+ * by page and its mallocs by size (unless tuning->no_partition).  The workgroup form
  * measured faster at every fill tried; the multi-heap driver autotunes the
  * form and the workgroup size. */
 int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tuning, void* stream);

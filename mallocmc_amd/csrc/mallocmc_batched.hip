@@ -274,8 +274,10 @@ namespace
             const u32 tile = nextTile(d.d_tile_cursor);
             if(tile >= d.n_tiles)
                 break;
-            ++nTiles;
             const u32 first = __builtin_amdgcn_readfirstlane(d.d_tiles[tile]);
+            if(first == MMC_NO_TILE)
+                continue; // padding
+            ++nTiles;
             const u32 heapIdx = __builtin_amdgcn_readfirstlane(d.d_slot_heap[first]);
             const u32 s = first + me;
             const bool valid = s < d.n_slots && d.d_slot_heap[s] == heapIdx;
@@ -339,15 +341,21 @@ namespace
 
     //! One step over many heaps, one workgroup per B/64 consecutive 64-slot
     //! tiles (no work queue).  The draws of a slot are those of
-    //! multiStepKernel.  When all its tiles lie in one heap the workgroup
-    //! first reorders its frees by page and its mallocs by size
+    //! multiStepKernel.  When all its tiles with slots lie in one heap the
+    //! workgroup first reorders its frees by page and its mallocs by size
     //! (partitionByKey), as the one-heap batch kernels do; the barriers all
     //! come before the wave calls, so a slow wave holds up no other.  The
     //! last workgroup to finish resets the finish counter and advances the
     //! step.
+    //! Eight waves per SIMD (64 VGPRs, two 1,024-thread workgroups per CU):
+    //! the step's waves mostly wait on atomics, and on the 64-heap workload
+    //! two workgroups per CU beat one without spills (1.44e9 against 1.09e9
+    //! mallocs+frees/s at 1,024 threads; 512 threads: 1.26e9 against 0.94e9
+    //! at the compiler's own 82 VGPRs), the spill of about 27 registers
+    //! notwithstanding.
     template<class Dist, u32 B>
-    __global__ __launch_bounds__(B) void multiStepBlockKernel(const ScatterHeap* heaps, mmc_step_desc d,
-                                                              WaveTuning t, bool partition)
+    __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) void multiStepBlockKernel(
+        const ScatterHeap* heaps, mmc_step_desc d, WaveTuning t, bool partition)
     {
         constexpr u32 W = B / 64u;
         __shared__ u32 sHeap[W];
@@ -363,8 +371,11 @@ namespace
         const u32 me = hw::laneid();
         const u32 w = threadIdx.x / 64u;
         const u32 tile = W * blockIdx.x + w;
-        const bool haveTile = tile < d.n_tiles;
-        const u32 first = haveTile ? __builtin_amdgcn_readfirstlane(d.d_tiles[tile]) : 0u;
+        // padding entries (MMC_NO_TILE) give a wave no slots of its own; it
+        // still serves its share of a partitioned workgroup's requests
+        const u32 entry = tile < d.n_tiles ? __builtin_amdgcn_readfirstlane(d.d_tiles[tile]) : MMC_NO_TILE;
+        const bool haveTile = entry != MMC_NO_TILE;
+        const u32 first = haveTile ? entry : 0u;
         const u32 heapIdx = haveTile ? __builtin_amdgcn_readfirstlane(d.d_slot_heap[first]) : 0xffffffffu;
         if(me == 0u)
         {
@@ -374,13 +385,27 @@ namespace
         if(threadIdx.x < 4u)
             sCount[threadIdx.x] = 0u;
         __syncthreads();
-        bool blockWide = partition && sHeap[0] != 0xffffffffu;
-        for(u32 i = 1; i < W; ++i)
-            blockWide = blockWide && sHeap[i] == sHeap[0];
+        // the workgroup's heap: that of its first tile with slots; the
+        // workgroup partitions its requests when all its tiles with slots
+        // lie in that heap
+        u32 blockHeap = 0xffffffffu;
+        bool oneHeap = true;
+        for(u32 i = 0; i < W; ++i)
+        {
+            if(sHeap[i] == 0xffffffffu)
+                continue;
+            if(blockHeap == 0xffffffffu)
+                blockHeap = sHeap[i];
+            else
+                oneHeap = oneHeap && sHeap[i] == blockHeap;
+        }
+        const bool blockWide = partition && blockHeap != 0xffffffffu && oneHeap;
         const u32 s = first + me;
         const bool valid = haveTile && s < d.n_slots && d.d_slot_heap[s] == heapIdx;
         // the view is the same for the whole wave: keep it in scalar registers
-        const ScatterHeap h = uniformCopy(heaps[__builtin_amdgcn_readfirstlane(haveTile ? heapIdx : sHeap[0])]);
+        // (a workgroup of padding only reads heap 0 and does nothing with it)
+        const u32 viewIdx = haveTile ? heapIdx : (blockHeap != 0xffffffffu ? blockHeap : 0u);
+        const ScatterHeap h = uniformCopy(heaps[__builtin_amdgcn_readfirstlane(viewIdx)]);
         const WaveAlloc wa{h, t};
         const u64 r = mix64(d.seed ^ mix64(step * 0x100000001B3ull + s));
         const u32 coin = static_cast<u32>(r % 1000u);

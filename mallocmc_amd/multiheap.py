@@ -35,13 +35,27 @@ class MultiWorkload:
     cdf: np.ndarray
 
 
-def make_workload(n_heaps: int = 64, seed: int = 11, lo: int = 1024, hi: int = 262144, s: float = 1.2):
+#: d_tiles entry that covers no slot (MMC_NO_TILE of include/mallocmc_batched.h)
+NO_TILE = 0xFFFFFFFF
+
+
+def make_workload(n_heaps: int = 64, seed: int = 11, lo: int = 1024, hi: int = 262144, s: float = 1.2,
+                  pad: int = 16):
+    """`pad`: each heap's tile list is padded with NO_TILE entries to a
+    multiple of `pad` tiles, so that no workgroup of the workgroup form (4, 8
+    or 16 tiles) straddles two heaps and every one partitions its requests;
+    1 pads nothing."""
     rng = np.random.default_rng(seed)
     u = rng.random(n_heaps)
     slots = np.floor(lo * (hi / lo) ** u).astype(np.int64)
     slot_heap = np.repeat(np.arange(n_heaps, dtype=np.uint32), slots)
     starts = np.concatenate([[0], np.cumsum(slots)[:-1]])
-    tiles = np.concatenate([np.arange(st, st + n, 64) for st, n in zip(starts, slots)]).astype(np.uint32)
+    per_heap = []
+    for st, n in zip(starts, slots):
+        t = np.arange(st, st + n, 64, dtype=np.uint64)
+        extra = (-len(t)) % pad
+        per_heap.append(np.concatenate([t, np.full(extra, NO_TILE, dtype=np.uint64)]))
+    tiles = np.concatenate(per_heap).astype(np.uint32)
     table = wl.ZIPF_SIZES
     w = 1.0 / np.arange(1, len(table) + 1) ** s
     cdf = np.minimum(np.round(np.cumsum(w / w.sum()) * 2 ** 32), 2 ** 32 - 1).astype(np.uint64)

@@ -202,7 +202,10 @@ def test_wave_multipage_fill_to_null(mmc, cuda):
     a.close()
 
 
-def _step_setup(torch, mmc, nheaps, slots_per_heap, heap_bytes, pagesize, seed=7, nsizes=8, distribution=None):
+def _step_setup(torch, mmc, nheaps, slots_per_heap, heap_bytes, pagesize, seed=7, nsizes=8, distribution=None,
+                pad=1):
+    """`pad`: each heap's tile list is padded with MMC_NO_TILE entries to a
+    multiple of `pad` tiles."""
     cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=heap_bytes // 4, regionsize=16,
                             resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK,
                             distribution=mmc.DIST_NOOP if distribution is None else distribution)
@@ -212,14 +215,15 @@ def _step_setup(torch, mmc, nheaps, slots_per_heap, heap_bytes, pagesize, seed=7
     tiles = []
     for hidx in range(nheaps):
         base = hidx * slots_per_heap
-        tiles += list(range(base, base + slots_per_heap, 64))
+        mine = list(range(base, base + slots_per_heap, 64))
+        tiles += mine + [0xFFFFFFFF] * ((-len(mine)) % pad)
     table = wl.ZIPF_SIZES[:nsizes]
     w = 1.0 / np.arange(1, len(table) + 1) ** 1.2
     cdf = np.minimum(np.round(np.cumsum(w / w.sum()) * 2 ** 32), 2 ** 32 - 1).astype(np.uint64)
     cdf[-1] = 2 ** 32 - 1
     dev = {
         "slot_heap": torch.from_numpy(slot_heap.astype(np.int32)).cuda(),
-        "tiles": torch.from_numpy(np.array(tiles, dtype=np.int32)).cuda(),
+        "tiles": torch.from_numpy(np.array(tiles, dtype=np.uint32).view(np.int32)).cuda(),
         "table": torch.from_numpy(table.astype(np.int32)).cuda(),
         "cdf": torch.from_numpy(cdf.astype(np.uint32).view(np.int32)).cuda(),
         "offs": torch.full((len(slot_heap),), -1, dtype=torch.int64, device="cuda"),
@@ -276,22 +280,27 @@ def _check_live(torch, allocs, dev, slot_heap):
         assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0, r
 
 
-@pytest.mark.parametrize("persistent,block_threads,per",
-                         [(True, 0, 4096), (False, 256, 4096), (False, 1024, 4096), (False, 1024, 1500)])
-def test_multi_heap_steps_and_graph_replay(mmc, cuda, persistent, block_threads, per):
+@pytest.mark.parametrize("persistent,block_threads,per,pad",
+                         [(True, 0, 4096, 1), (False, 256, 4096, 1), (False, 1024, 4096, 1), (False, 1024, 1500, 1),
+                          (False, 1024, 1500, 16), (False, 256, 1500, 16), (True, 0, 1500, 16)])
+def test_multi_heap_steps_and_graph_replay(mmc, cuda, persistent, block_threads, per, pad):
     """8 heaps, 20 steps eager, then 20 steps replayed from a captured graph;
     the live set follows the seeded coins exactly and free-all restores every
     heap's slot count.  Both launch forms: persistent waves pulling tiles, and
     one workgroup per 4 or 16 tiles, which reorders a one-heap workgroup's
     requests by page and size.  With 4096 slots per heap every workgroup lies
     in one heap; with 1500 (24 tiles, the last one ragged) workgroups of 16
-    tiles straddle two heaps and serve them unpartitioned."""
+    tiles straddle two heaps and serve them unpartitioned, unless each heap's
+    tiles are padded to 16 with MMC_NO_TILE entries (which the persistent form
+    skips and a workgroup's padding waves serve its partitioned requests).
+    Tiles processed count only real tiles."""
     torch = cuda
     nheaps = 8
-    allocs, multi, dev, d, slot_heap = _step_setup(torch, mmc, nheaps, per, 64 * MiB, 16384)
+    allocs, multi, dev, d, slot_heap = _step_setup(torch, mmc, nheaps, per, 64 * MiB, 16384, pad=pad)
+    real_tiles = nheaps * ((per + 63) // 64)
     s0 = [a.getAvailableSlots(64) for a in allocs]
     tun = mmc.Tuning(persistent=persistent, aggregation=64, block_threads=block_threads)
-    print(f"multi-heap steps persistent={persistent} block_threads={block_threads}", flush=True)
+    print(f"multi-heap steps persistent={persistent} block_threads={block_threads} per={per} pad={pad}", flush=True)
     live_hist = []
     for _ in range(20):
         multi.step(d, tun)
@@ -299,6 +308,7 @@ def test_multi_heap_steps_and_graph_replay(mmc, cuda, persistent, block_threads,
     torch.cuda.synchronize()
     c = dev["counters"].cpu().numpy()
     assert c[2] == 0, "no nulls expected at this fill"
+    assert c[3] == 20 * real_tiles, (c[3], real_tiles)
     assert int(dev["step"].item()) == 20
     _check_live(torch, allocs, dev, slot_heap)
     # replay the coins on the host: placement may differ, the live set may not

@@ -2,21 +2,31 @@
 GPU: the workload's tiles, and the two-stage autotune's choice over a stand-in
 for the device run whose rate is a known function of the tuning."""
 import numpy as np
+import pytest
 
 from mallocmc_amd import multiheap as mh
 
 
-def test_workload_tiles_cover_each_heap_once():
-    work = mh.make_workload(n_heaps=8, lo=100, hi=1000)
+@pytest.mark.parametrize("pad", [1, 4, 16])
+def test_workload_tiles_cover_each_heap_once(pad):
+    work = mh.make_workload(n_heaps=8, lo=100, hi=1000, pad=pad)
     starts = np.concatenate([[0], np.cumsum(work.slots_per_heap)])
     # every slot of a heap lies in exactly one tile of that heap, tiles start 64 apart
     covered = np.zeros(len(work.slot_heap), dtype=np.int64)
     for t in work.tiles:
+        if t == mh.NO_TILE:
+            continue
         h = work.slot_heap[t]
         end = min(int(t) + 64, int(starts[h + 1]))
         covered[int(t):end] += 1
         assert (work.slot_heap[int(t):end] == h).all()
     assert (covered == 1).all()
+    # padded to `pad` entries per heap: every aligned group of `pad` entries
+    # (a workgroup of pad, or a divisor of pad, waves) lies in one heap
+    assert len(work.tiles) % pad == 0
+    for g in range(0, len(work.tiles), pad):
+        hs = {int(work.slot_heap[t]) for t in work.tiles[g:g + pad] if t != mh.NO_TILE}
+        assert len(hs) <= 1, (g, hs)
     assert work.cdf[-1] == 2 ** 32 - 1 and (np.diff(work.cdf.astype(np.int64)) >= 0).all()
 
 
