@@ -68,7 +68,8 @@ def build(verbose: bool = False) -> str:
 
 
 CPP_DIR = os.path.join(REPO_DIR, "tests", "cpp")
-CPP_PROGRAMS = ["example_arrays", "example_slots", "verify_heap", "thread_malloc", "custom_policies"]
+CPP_PROGRAMS = ["example_arrays", "example_slots", "verify_heap", "thread_malloc", "custom_policies",
+                "udiv24_exhaustive"]
 
 
 def build_cpp_programs(verbose: bool = False) -> list:

@@ -106,7 +106,8 @@ def test_cpp_programs_built():
     """The drop-in C++ headers compile into the example programs (built by build())."""
     import os
     d = os.path.join(REPO, "tests", "cpp", "_build")
-    for prog in ("example_arrays", "example_slots", "verify_heap", "thread_malloc", "custom_policies"):
+    for prog in ("example_arrays", "example_slots", "verify_heap", "thread_malloc", "custom_policies",
+                 "udiv24_exhaustive"):
         assert os.path.exists(os.path.join(d, prog)), prog
 
 

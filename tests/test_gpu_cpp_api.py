@@ -47,6 +47,15 @@ def test_user_written_policies():
     assert r.returncode == 0 and "custom_policies OK" in r.stdout
 
 
+def test_udiv24_exhaustive():
+    """The float-reciprocal division of pdiv equals n / d for every n <= 65536
+    and every d in 1..65536 (every pagesize up to 64 KiB, every divisor the
+    heap uses), and on 2^28 sampled pairs below 2^24 wherever the quotient
+    is below 2^22, and clamped to 32 always."""
+    r = _run("udiv24_exhaustive")
+    assert r.returncode == 0 and "udiv24 OK" in r.stdout
+
+
 def test_thread_malloc_drop_in():
     """1M threads malloc and free through the drop-in handle: full
     wavefronts take the wave-cooperative path, wavefronts with a lane missing
