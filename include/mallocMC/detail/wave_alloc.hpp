@@ -1004,6 +1004,98 @@ namespace mallocMC
                 return group & ~need;
             }
 
+            //! Marks pages [sp, sp + k) as one run of `bytes` (chunk size =
+            //! bytes on every page), 64 pages per round trip: lane l CASes
+            //! pages sp + l, sp + l + 64, ... from 0 to bytes in order and
+            //! stops at its first page already taken; if any lane met one,
+            //! every lane sets the pages it marked back to 0.  Only its own:
+            //! a page that another claim of the same `bytes` holds reads the
+            //! same value, and resetting it would free a run that claim won.
+            //! The end state is that of the reference's markpages
+            //! (Scatter.hpp:745-764), which CASes the pages one by one and
+            //! rolls back the ones before the first taken page.
+            MALLOCMC_D bool markpagesWave(u32 sp, u32 k, u32 bytes) const
+            {
+                const u32 me = hw::laneid();
+                u32 marked = 0u; // this lane's pages sp + me + 64 * x, x < marked, hold bytes
+                bool lost = false;
+                for(u32 i = me; i < k; i += 64u)
+                {
+                    if(atom::cas(h.pteChunksize(sp + i), 0u, bytes) != 0u)
+                    {
+                        lost = true;
+                        break;
+                    }
+                    ++marked;
+                }
+                if(hw::ballot(lost) == 0u)
+                    return true;
+                for(u32 x = 0; x < marked; ++x)
+                    atom::cas(h.pteChunksize(sp + me + 64u * x), bytes, 0u);
+                return false;
+            }
+
+            //! A run of k > 32 pages (the reference serializes every run on
+            //! one mutex, Scatter.hpp:824-840).  Same top-down order from
+            //! _firstFreePageBased as allocRun, one 64-page window per round
+            //! trip without overlap: lane i learns how many free pages start
+            //! at its page (the free pages above it in the window, plus the
+            //! free run the window above ended with when its own run reaches
+            //! the window's top), and the highest page with k or more is
+            //! claimed with markpagesWave.  The cursor is not moved for such
+            //! runs: the per-lane path scans page by page from it and needs
+            //! every page above it in use, and a run across windows leaves no
+            //! cheap proof that no shorter run lies above it.
+            MALLOCMC_D char* allocLongRun(u32 bytes, u32 k) const
+            {
+                const u32 me = hw::laneid();
+                const u32 numpages = h.numpages;
+                if(k > numpages)
+                    return nullptr;
+                u32 start = 0u;
+                if(me == 0u)
+                    start = atom::load(&h.scalars->firstFreePageBased);
+                start = wave::lane(start, 0u);
+                if(start >= numpages)
+                    start = numpages - 1u;
+                for(u32 sweep = 0; sweep < 2u; ++sweep)
+                {
+                    long long top = sweep == 0u ? static_cast<long long>(start) : static_cast<long long>(numpages) - 1;
+                    const long long bottom = sweep == 0u ? 0 : static_cast<long long>(start);
+                    // free pages starting at top + 1 and going up (only what this
+                    // sweep has seen: the first window of a sweep starts with 0)
+                    u32 runAbove = 0u;
+                    while(top >= bottom)
+                    {
+                        const long long base = top - 63 > bottom ? top - 63 : bottom;
+                        const u32 width = static_cast<u32>(top - base) + 1u;
+                        const long long pg = base + me;
+                        bool isFree = false;
+                        if(me < width)
+                            isFree = atom::load(h.pteChunksize(static_cast<u32>(pg))) == 0u;
+                        const u64 m = hw::ballot(isFree);
+                        // free pages from this lane's page up to the window's top
+                        const u64 upFromMe = me < width ? (m >> me) : 0u;
+                        const u32 ones = upFromMe == ~0ull ? 64u : static_cast<u32>(__builtin_ctzll(~upFromMe));
+                        const u32 len = isFree ? (ones >= width - me ? ones + runAbove : ones) : 0u;
+                        u64 cand = hw::ballot(isFree && len >= k);
+                        while(cand != 0u)
+                        {
+                            const u32 pos = 63u - static_cast<u32>(__builtin_clzll(cand));
+                            const u32 sp = static_cast<u32>(base) + pos;
+                            if(sp + k <= numpages && markpagesWave(sp, k, bytes))
+                                return h.pageData(sp);
+                            cand &= ~(1ull << pos);
+                        }
+                        runAbove = wave::lane(len, 0u); // the free run that starts at base
+                        if(base == bottom)
+                            break;
+                        top = base - 1;
+                    }
+                }
+                return nullptr;
+            }
+
             //! Multi-page request of `bytes` > pagesize for one lane, served by the
             //! whole wave (uniform `bytes`).  The reference takes a global spin
             //! mutex and scans one PTE at a time from _firstFreePageBased down
@@ -1018,13 +1110,7 @@ namespace mallocMC
                 const u32 numpages = h.numpages;
                 const u32 k = divup(bytes, P);
                 if(k > 32u)
-                {
-                    // very long runs: the reference's serialized path on one lane
-                    u64 r = 0u;
-                    if(me == 0u)
-                        r = reinterpret_cast<u64>(h.allocPageBasedSingle(bytes));
-                    return reinterpret_cast<char*>(hw::readlane64(r, 0u));
-                }
+                    return allocLongRun(bytes, k);
                 u32 start = 0u;
                 if(me == 0u)
                     start = atom::load(&h.scalars->firstFreePageBased);
@@ -1067,7 +1153,9 @@ namespace mallocMC
                                 // (Scatter.hpp:791-798).  Runs claimed concurrently
                                 // leave the cursor behind the pages they took, so
                                 // lower it with a min instead of the reference's
-                                // CAS from the value read.
+                                // CAS from the value read.  The per-lane path
+                                // relies on this: it scans page by page from the
+                                // cursor down, so no free run may lie above it.
                                 const u64 above = pos + k < 64u ? (pairs >> (pos + k)) : 0u;
                                 if(me == 0u && usedAbove && above == 0u && sp > 0u)
                                     atom::min(&h.scalars->firstFreePageBased, sp - 1u);
@@ -1116,7 +1204,7 @@ namespace mallocMC
                     if(kl < failedPages)
                     {
                         r = allocRun(bl);
-                        if(r == nullptr && kl <= 32u)
+                        if(r == nullptr)
                             failedPages = kl;
                     }
                     if(me == l)

@@ -57,7 +57,14 @@ namespace mallocMC
          * distribution policy has a wave device type, they are served by the
          * wave-cooperative path (malloc_wave): same heap protocol, one group
          * claim per size class instead of one reference walk per lane.
-         * Otherwise each lane runs the reference's composition (mallocLane). */
+         * Otherwise each lane runs the reference's composition (mallocLane).
+         * Requests above a page stay on the reference's page-based path: in
+         * a kernel where some wavefronts are full and others are not (lanes
+         * that leave an allocate-until-null loop at their own null), the
+         * per-lane page-by-page scan needs the page-based cursor exact --
+         * every page above it in use -- which concurrent wave run claims
+         * cannot keep; with wave runs for the full wavefronts, 128 waves
+         * filling 1 GiB with 2-page runs took over 150 s. */
         template<typename Acc>
         __device__ void* malloc(const Acc& acc, std::size_t bytes) const
         {

@@ -15,7 +15,10 @@
 //   3. the same OOM policy behind the built-in Noop distribution, where full
 //      wavefronts take the wave-cooperative path: 64 lanes malloc(4 KiB) on a
 //      heap with fewer pages than lanes; the nulls are counted again;
-//   4. frees restore the slot counts; Allocator::info() names both policies.
+//   4. frees restore the slot counts; Allocator::info() names both policies;
+//   5. a full wavefront asking for 40-page runs: the drop-in serves
+//      requests above a page per lane (the reference's page-based path),
+//      every run distinct and the pages restored.
 // Prints "custom_policies OK".
 #include <mallocMC/mallocMC.hpp>
 
@@ -277,6 +280,32 @@ int main()
         if(freeAll(a, q, dPtrs, 64u, dBad))
             return 1;
         CHECK(a.getAvailableSlots(dev, q, 4096) == pages, "pages not restored");
+    }
+
+    {
+        // 5. requests above a page through the drop-in: 64 lanes of one
+        //    wavefront each ask for 40 pages
+        BuiltinDistribution a(dev, q, 16u * 1024u * 1024u);
+        const unsigned s4k = a.getAvailableSlots(dev, q, 4096);
+        const unsigned bytes = 40u * 4096u - 64u;
+        if(runAndFree(a, q, dPtrs, 64u, bytes, ptrs, dBad))
+            return 1;
+        std::vector<char*> sorted;
+        for(void* p : ptrs)
+        {
+            CHECK(p != nullptr, "a 40-page request failed on a heap of %u free pages", s4k);
+            sorted.push_back(static_cast<char*>(p));
+        }
+        std::sort(sorted.begin(), sorted.end());
+        for(std::size_t i = 1; i < sorted.size(); ++i)
+            CHECK(sorted[i] - sorted[i - 1] >= static_cast<std::ptrdiff_t>(bytes), "runs %zu and %zu overlap", i - 1,
+                  i);
+        CHECK(a.getAvailableSlots(dev, q, 4096) == s4k - 64u * 40u, "free pages %u -> %u", s4k,
+              a.getAvailableSlots(dev, q, 4096));
+        if(freeAll(a, q, dPtrs, 64u, dBad))
+            return 1;
+        CHECK(a.getAvailableSlots(dev, q, 4096) == s4k, "pages not restored after the runs");
+        std::printf("40-page runs: 64 of 64 served\n");
     }
 
     HIP(hipFree(dPtrs));

@@ -344,6 +344,69 @@ def test_multi_heap_steps_and_graph_replay(mmc, cuda, persistent, block_threads,
         a.close()
 
 
+def _longest_free_run(free):
+    best = cur = 0
+    for f in free:
+        cur = cur + 1 if f else 0
+        best = max(best, cur)
+    return best
+
+
+def test_wave_long_runs(mmc, cuda):
+    """Runs of more than 32 pages on the wave path, without the page-based
+    mutex: 4 KiB pages, 33..257-page requests mixed with small ones over 6
+    waves on a 256 MiB heap with room to spare: no null, no overlap, the
+    pattern intact, the metadata equal to the live list, slots restored.
+    Then one wave asks for 1,500-page runs until the heap is full: it claims
+    them one after another from the top, so it takes exactly numpages // 1500
+    runs and leaves no 1,500 free pages in a row."""
+    from mallocmc_amd import snapshot
+    torch = cuda
+    cfg = mmc.ScatterConfig(pagesize=4096, accessblocksize=256 * MiB, regionsize=16, resetfreedpages=True)
+    a = mmc.ScatterAllocator(cfg, 256 * MiB)
+    g = a.geometry()
+    big = 1500
+    s0 = [a.getAvailableSlots(x) for x in (64, 4096, big * 4096)]
+    pages = np.array([33, 40, 64, 65, 100, 130, 257, 1], dtype=np.int64)
+    n = 384
+    k = pages[np.arange(n) % len(pages)]
+    sizes_np = np.where(k == 1, 48, k * 4096 - 100).astype(np.int32)
+    assert int(k.sum()) < 0.6 * int(g.numpages)
+    sizes = torch.from_numpy(sizes_np).cuda()
+    print(f"long runs: {g.numpages} pages, {n} requests, {int(k.sum())} pages asked", flush=True)
+    offs = a.wave_malloc_batch(sizes)
+    assert int((offs < 0).sum().item()) == 0
+    a.write_pattern(offs, sizes, salt=5)
+    r = a.verify_pattern(offs, sizes, salt=5)
+    assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0, r
+    assert no_overlap(torch, offs, sizes)
+    padded = ((sizes.to(torch.int64) + 15) // 16) * 16
+    rep = snapshot.verify(snapshot.take(a), offs, padded, complete=True)
+    assert rep["ok"], rep["message"]
+    a.wave_free_batch(offs)
+    torch.cuda.synchronize()
+    assert [a.getAvailableSlots(x) for x in (64, 4096, big * 4096)] == s0
+    # one wave, 1,500-page runs until null
+    assert int(g.numpages) // big < 64
+    sizes = torch.full((64,), big * 4096, dtype=torch.int32, device="cuda")
+    offs = a.wave_malloc_batch(sizes)
+    served = int((offs >= 0).sum().item())
+    snap = snapshot.take(a)
+    hdr = snapshot.header(snap)
+    pte = np.frombuffer(bytes(snap[hdr.off_ptes: hdr.off_ptes + 12 * int(g.numpages)]), dtype=np.uint32)
+    free = pte.reshape(-1, 3)[:, 0] == 0
+    print(f"  {big}-page runs: served {served}, longest free run {_longest_free_run(free)}", flush=True)
+    assert served == int(g.numpages) // big
+    assert _longest_free_run(free) < big
+    assert no_overlap(torch, offs, sizes)
+    rep = snapshot.verify(snap, offs, sizes.to(torch.int64), complete=True)
+    assert rep["ok"], rep["message"]
+    a.wave_free_batch(offs)
+    torch.cuda.synchronize()
+    assert [a.getAvailableSlots(x) for x in (64, 4096, big * 4096)] == s0
+    a.close()
+
+
 def test_multi_heap_steps_with_multipage_requests(mmc, cuda):
     """Pagesize 4 KiB with sizes up to 64 KiB: multi-page runs (markpages) and
     chunk claims race on the same free pages in one launch.  A claim that
