@@ -62,22 +62,26 @@ def _no_overlap(torch, offs, sizes):
 
 def run_default(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles: int = 5, pagesize: int = 65536,
                 regionsize: int = 32, tuning: mmc.Tuning | None = None, log=None, cap: int | None = None,
-                first_null: bool = True, verify_snapshots: bool = False) -> dict:
+                first_null: bool = True, verify_snapshots: bool = False, accessblocksize: int | None = None) -> dict:
     """first_null: every lane stops at the first null any lane sees (the
     heap's fill at its first failure); else each lane allocates until its own
     null (verify_heap's AllocAll).  verify_snapshots: after every fill, also
     check a snapshot of the heap's metadata against the complete list of live
     allocations (mallocmc_amd/snapshot.py).
 
-    Defaults: regionsize 32 and one size class per octave (class_split 0).
-    Measured on the 64 GiB heap, 4 cycles each: refill 0.17-0.20 s at
-    regionsize 32 / split 0, 0.21-0.24 s at split 1, 0.29-0.34 s at
-    regionsize 16 / split 2, 0.22-0.26 s at regionsize 64 / split 0; the fill
-    at the first null stays 0.69-0.71 at all of them."""
+    Defaults: one access block over the whole heap (`accessblocksize` None),
+    regionsize 32 and one size class per octave (class_split 0).  Every walk
+    starts at its hashed page within the access block under the cursor; with
+    2 GiB blocks all of them start in the same 32768 pages and meet at the
+    same last free pages as the heap fills.  Measured on the 64 GiB heap, 4
+    cycles each, refill: one access block 0.024-0.028 s, 8 GiB blocks
+    0.19-0.21 s, 2 GiB blocks 0.21-0.27 s (regionsize 32; at 2 GiB blocks
+    regionsize 16 with four classes per octave 0.29-0.34 s).  The fill at the
+    first null stays 0.68-0.71 at all of them, with no page left free."""
     import torch
     L = _bind()
     tuning = tuning or mmc.Tuning(aggregation=64, class_split=0)
-    cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=min(2 * GiB, heap_bytes), regionsize=regionsize,
+    cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=accessblocksize or heap_bytes, regionsize=regionsize,
                             resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
     alloc = mmc.ScatterAllocator(cfg, heap_bytes)
     g = alloc.geometry()
@@ -148,5 +152,6 @@ def run_default(heap_bytes: int = 64 * GiB, lanes: int = 2 * 1024 * 1024, cycles
                     f"20% U[8,60] KiB, free a random half, {cycles} cycles",
         "cycles": results, "invariants_ok": bool(ok), "slots_restored": slots_end == slots0,
         "mean_fill_fraction": float(np.mean([r["fill_fraction"] for r in results])),
-        "tuning": dict(aggregation=tuning.aggregation, class_split=tuning.class_split, regionsize=regionsize),
+        "tuning": dict(aggregation=tuning.aggregation, class_split=tuning.class_split, regionsize=regionsize,
+                       accessblocksize=int(cfg.accessblocksize)),
     }
