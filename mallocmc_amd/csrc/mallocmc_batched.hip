@@ -79,47 +79,47 @@ namespace
 
     //! Reorders the B values of a block (B = 256, 512 or 1024 threads) by an
     //! 8-bit key (LDS counting sort, unstable).  Returns the value this
-    //! thread receives and the block slot it came from.
+    //! thread receives and the block slot it came from.  Three barriers:
+    //! after clearing the bucket counts, after counting, after scattering.
+    //! Between the last two, every wave scans the 256 counts itself and
+    //! writes the same bucket starts to LDS (a wave reads only after its own
+    //! writes, and every wave writes the same values), instead of four waves
+    //! scanning and two more barriers to share their totals.
     template<class T, u32 B>
     __device__ void partitionByKey(u32 key, T value, T& outValue, u32& outSlot)
     {
         static_assert(B % 256u == 0u && B <= 1024u, "a block of 256, 512 or 1024 threads");
+        static_assert(kSizeBuckets == 4u * 64u, "four buckets per lane in the scan");
         __shared__ u32 count[kSizeBuckets];
         __shared__ u32 start[kSizeBuckets];
-        __shared__ u32 waveTotal[4];
         __shared__ T sortedValue[B];
         __shared__ u32 sortedSlot[B];
         const u32 tid = threadIdx.x;
+        const u32 me = hw::laneid();
         if(tid < kSizeBuckets)
             count[tid] = 0u;
         __syncthreads();
         const u32 rank = atomicAdd(&count[key], 1u);
         __syncthreads();
-        // exclusive scan of the 256 bucket counts by the first four waves:
-        // per wave, then across them
-        u32 c = 0u, incl = 0u;
-        if(tid < kSizeBuckets)
+        // lane l: buckets 4l .. 4l+3, then an inclusive scan of the lane sums
+        const u32 c0 = count[4u * me], c1 = count[4u * me + 1u], c2 = count[4u * me + 2u], c3 = count[4u * me + 3u];
+        const u32 sum = c0 + c1 + c2 + c3;
+        u32 incl = sum;
+        for(u32 d = 1u; d < 64u; d <<= 1)
         {
-            c = count[tid];
-            incl = c;
-            for(u32 d = 1u; d < 64u; d <<= 1)
-            {
-                const u32 up = __shfl_up(incl, d, 64);
-                if(hw::laneid() >= d)
-                    incl += up;
-            }
-            if(hw::laneid() == 63u)
-                waveTotal[tid / 64u] = incl;
+            const u32 up = __shfl_up(incl, d, 64);
+            if(me >= d)
+                incl += up;
         }
-        __syncthreads();
-        if(tid < kSizeBuckets)
-        {
-            u32 before = 0u;
-            for(u32 w = 0; w < tid / 64u; ++w)
-                before += waveTotal[w];
-            start[tid] = before + incl - c;
-        }
-        __syncthreads();
+        const u32 excl = incl - sum;
+        start[4u * me] = excl;
+        start[4u * me + 1u] = excl + c0;
+        start[4u * me + 2u] = excl + c0 + c1;
+        start[4u * me + 3u] = excl + c0 + c1 + c2;
+        // this wave's writes are in LDS before its reads (LDS operations of a
+        // wave complete in order; the fence keeps the compiler from moving
+        // the reads up)
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const u32 pos = start[key] + rank;
         sortedValue[pos] = value;
         sortedSlot[pos] = tid;
