@@ -144,6 +144,14 @@ namespace mallocMC
             {
                 return __builtin_amdgcn_readlane(v, l);
             }
+            //! A value every lane holds alike, marked as such: the compiler
+            //! cannot always prove it (the claim counts), and a loop that exits
+            //! on a value it takes for divergent keeps its live values in
+            //! vector registers (16 more in the wave malloc).
+            MALLOCMC_D u32 uniform(u32 v)
+            {
+                return static_cast<u32>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+            }
             //! Position (0-based) of the r-th set bit of x; x must have > r bits.
             MALLOCMC_D u32 selectBit(u32 x, u32 r)
             {
@@ -186,6 +194,104 @@ namespace mallocMC
             const u32 j = 31u - static_cast<u32>(__builtin_clz(q));
             const u32 sub = j >= split ? ((q - (1u << j)) >> (j - split)) : (q - (1u << j));
             return 1u + ((j << 8) | sub);
+        }
+
+        //! Rank of a request slot that is not in the group being served.
+        constexpr u32 kNoRank = 0xffffffffu;
+
+        //! The requests a claim serves: every lane holds M request slots, and
+        //! rank[j] is slot j's rank among the requests still waiting
+        //! (kNoRank, or any value at least the claim's count, for a slot
+        //! that waits for nothing).  Claims hand out chunks in rank order,
+        //! so a claim that served n chunks served ranks 0 .. n-1.
+        template<u32 M>
+        struct GroupRanks
+        {
+            u32 rank[M];
+        };
+
+        //! The waiting requests of a group of one request per lane: the
+        //! lanes in `bits`, ranked by lane (uniform 64-bit state).
+        struct LaneNeed
+        {
+            u64 bits;
+
+            MALLOCMC_D bool open() const
+            {
+                return bits != 0u;
+            }
+            MALLOCMC_D u32 remaining() const
+            {
+                return static_cast<u32>(hw::popc(bits));
+            }
+            //! Progress marker: changes whenever a claim serves a request.
+            MALLOCMC_D u32 mark() const
+            {
+                return remaining();
+            }
+            MALLOCMC_D GroupRanks<1> ranks() const
+            {
+                GroupRanks<1> r;
+                r.rank[0] = ((bits >> hw::laneid()) & 1u) != 0u ? hw::rankBelow(bits) : kNoRank;
+                return r;
+            }
+            //! The n lowest-ranked waiting requests were served.
+            MALLOCMC_D void served(u32 n)
+            {
+                const GroupRanks<1> r = ranks();
+                bits &= ~hw::ballot(r.rank[0] < n);
+            }
+        };
+
+        //! The waiting requests of a group of M requests per lane: absolute
+        //! ranks 0 .. K-1 (`rank`), of which the lowest `got` are served.
+        template<u32 M>
+        struct RankNeed
+        {
+            GroupRanks<M> rank;
+            u32 K;
+            u32 got;
+
+            MALLOCMC_D bool open() const
+            {
+                return got < K;
+            }
+            MALLOCMC_D u32 remaining() const
+            {
+                return K - got;
+            }
+            MALLOCMC_D u32 mark() const
+            {
+                return got;
+            }
+            MALLOCMC_D GroupRanks<M> ranks() const
+            {
+                // below `got` and kNoRank wrap to values no claim reaches
+                GroupRanks<M> r;
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                    r.rank[j] = rank.rank[j] - got;
+                return r;
+            }
+            MALLOCMC_D void served(u32 n)
+            {
+                got += n;
+            }
+        };
+
+        //! Hands the chunks of `bits` (in bit order, chunk size cs, page data
+        //! at `base`) to the group's ranks [from, from + n).
+        template<u32 M>
+        MALLOCMC_D void handOut(const GroupRanks<M>& g, u32 from, u32 n, char* base, u32 bits, u32 cs,
+                                char* (&out)[M])
+        {
+#pragma unroll
+            for(u32 j = 0; j < M; ++j)
+            {
+                const u32 r = g.rank[j] - from; // wraps to a huge value below `from` and for kNoRank
+                if(r < n)
+                    out[j] = base + static_cast<u64>(wave::selectBit(bits, r)) * cs;
+            }
         }
 
         struct WaveAlloc
@@ -265,10 +371,11 @@ namespace mallocMC
                     resetEmpty(page);
             }
 
-            //! Claims up to popc(need) chunks of size cs on `page` for the group
-            //! lanes in `need` (a subset of the wave).  Returns the lanes served;
-            //! each served lane's `out` gets its chunk.  `fresh`: the scan saw
-            //! the page unformatted, so the claim formats it (CAS 0 -> cs).
+            //! Claims up to k chunks of size cs on `page` for the waiting ranks
+            //! 0 .. k-1 of `g`.  Returns how many it served, lowest ranks
+            //! first; each served slot's `out` gets its chunk.  `fresh`: the
+            //! scan saw the page unformatted, so the claim formats it (CAS
+            //! 0 -> cs).
             //!
             //! The reference's tryUsePage (Scatter.hpp:469-517) reserves with
             //! atomicAdd(count), then re-checks the chunk size with a CAS, then
@@ -283,8 +390,9 @@ namespace mallocMC
             //! reservation lands.  When the reservation finds the page
             //! unformatted, the claim formats it afterwards, as the reference
             //! does.
-            MALLOCMC_D u64 claimOnPage(u32 page, u32 cs, u32 bmax, u32 maxcs, u64 need, bool fresh,
-                                       char*& out) const
+            template<u32 M>
+            MALLOCMC_D u32 claimOnPage(u32 page, u32 cs, u32 bmax, u32 maxcs, u32 k, const GroupRanks<M>& g,
+                                       bool fresh, char* (&out)[M]) const
             {
                 const u32 me = hw::laneid();
                 const u32 P = h.pagesize;
@@ -292,7 +400,6 @@ namespace mallocMC
                 u32 waited = 0u;
                 for(;;)
                 {
-                    const u32 k = hw::popc(need);
                     u32 old = 0u, rcs = 0u, fcs = 1u;
                     if(me == 0u)
                     {
@@ -362,10 +469,8 @@ namespace mallocMC
                     }
                     const u32 cap = h.pageCapacity(cs);
                     const u32 grant = old < cap ? umin32(k, cap - old) : 0u;
-                    u32 got = 0u;
-                    u64 served = 0u;
-                    if(grant != 0u)
-                        served = claimBits(page, cs, L, grant, need, got, out, snapValid, snap);
+                    const u32 got = grant != 0u ? claimBits<M>(page, cs, L, grant, g, out, snapValid, snap)
+                                                : 0u;
                     if(got < k && me == 0u)
                     {
                         if(got == 0u)
@@ -373,26 +478,26 @@ namespace mallocMC
                         else
                             atom::sub(h.pteCount(page), k - got); // holds got chunks: count stays > 0
                     }
-                    return served;
+                    return got;
                 }
             }
 
             //! Claims `grant` free bits on the page's mask words and hands the
-            //! chunks to the lanes of `need` in rank order.  Returns lanes served.
-            //! Pass 0 chooses from `snap0` (lane w: word w) when `haveSnap`.
-            MALLOCMC_D u64 claimBits(u32 page, u32 cs, const Layout& L, u32 grant, u64 need, u32& gotTotal,
-                                     char*& out, bool haveSnap, u32 snap0) const
+            //! chunks to the waiting ranks of `g` in order.  Returns
+            //! how many it handed out.  Pass 0 chooses from `snap0` (lane w:
+            //! word w) when `haveSnap`.
+            template<u32 M>
+            MALLOCMC_D u32 claimBits(u32 page, u32 cs, const Layout& L, u32 grant, const GroupRanks<M>& g,
+                                     char* (&out)[M], bool haveSnap, u32 snap0) const
             {
                 const u32 me = hw::laneid();
                 const u32 P = h.pagesize;
                 const bool hier = L.hier;
                 const u32 nWords = L.nWords, full = L.full, additional = L.additional;
                 u32* const words = L.words;
-                const u32 myRank = hw::rankBelow(need);
-                const bool inNeed = ((need >> me) & 1u) != 0u;
-                u64 served = 0u;
+                char* const data = h.pageData(page);
                 u32 remaining = grant;
-                gotTotal = 0u;
+                u32 gotTotal = 0u;
                 for(u32 pass = 0; pass < 3u && remaining > 0u; ++pass)
                 {
                     // lane w < nWords snapshots mask word w
@@ -432,22 +537,16 @@ namespace mallocMC
                         words64 &= words64 - 1u;
                         const u32 bits = wave::lane(gotBits, w);
                         const u32 n = hw::popc(bits);
-                        if(inNeed && myRank >= gotTotal && myRank < gotTotal + n)
-                        {
-                            const u32 idx = 32u * w + wave::selectBit(bits, myRank - gotTotal);
-                            out = h.pageData(page) + static_cast<u64>(idx) * cs;
-                        }
+                        handOut<M>(g, gotTotal, n, data + static_cast<u64>(32u * w) * cs, bits, cs, out);
                         gotTotal += n;
                     }
                     remaining = grant - gotTotal;
                 }
-                // lanes of `need` with rank < gotTotal were served
-                served = hw::ballot(inNeed && myRank < gotTotal);
-                return served;
+                return gotTotal;
             }
 
-            //! Claims chunks for the lanes of `need` on several flat pages of one
-            //! region at once.  Lane l stands for page `pageL` of the scan
+            //! Claims chunks for the waiting ranks 0 .. k-1 of `g` on several flat
+            //! pages of one region at once.  Lane l stands for page `pageL` of the scan
             //! (chunk size pcs, count pcount, PTE bitmask pbits as the scan saw
             //! them); `used` / `fresh` are the candidate lanes.  Pages are
             //! taken in the scan's preference order (used pages from the
@@ -460,10 +559,13 @@ namespace mallocMC
             //! claimBits, so picking from a stale bitmask only costs chunks,
             //! never hands one out twice.  A page that turned out locked by a
             //! reset, or whose reservation saw another chunk size, gives back
-            //! what it won.  Returns the lanes served; `failed` gets the chosen
-            //! pages that yielded no chunk (the scan's view of them was stale).
-            MALLOCMC_D u64 claimFlat(u32 pageL, u32 pcs, u32 pcount, u32 pbits, u64 used, u64 fresh, u32 rot,
-                                     u32 c, u64 need, u64& chosen, u64& failed, char*& out) const
+            //! what it won.  Returns how many ranks it served, lowest first;
+            //! `failed` gets the chosen pages that yielded no chunk (the scan's
+            //! view of them was stale).
+            template<u32 M>
+            MALLOCMC_D u32 claimFlat(u32 pageL, u32 pcs, u32 pcount, u32 pbits, u64 used, u64 fresh, u32 rot,
+                                     u32 c, u32 k, const GroupRanks<M>& g, u64& chosen, u64& failed,
+                                     char* (&out)[M]) const
             {
                 const u32 me = hw::laneid();
                 const u32 P = h.pagesize;
@@ -497,7 +599,6 @@ namespace mallocMC
                     pos = tUhi + tUlo + pFhi;
                 else if(inFlo)
                     pos = tUhi + tUlo + tFhi + pFlo;
-                const u32 k = static_cast<u32>(hw::popc(need));
                 const u32 take = (f != 0u && pos < k) ? umin32(f, k - pos) : 0u;
                 const u32 choose = take != 0u ? wave::lowestBits(freeBits, take) : 0u;
                 chosen = hw::ballot(take != 0u);
@@ -546,94 +647,103 @@ namespace mallocMC
                         atom::sub(h.pteCount(pageL), take - kept); // holds kept chunks: count stays > 0
                 }
                 failed = hw::ballot(take != 0u && kept == 0u);
-                // hand the kept chunks to the group's lanes in rank order
-                const u32 myRank = hw::rankBelow(need);
-                const bool inNeed = ((need >> me) & 1u) != 0u;
+                // hand the kept chunks to the group's ranks in order
                 u32 got = 0u;
                 for(u64 pagesKept = hw::ballot(kept != 0u); pagesKept != 0u; pagesKept &= pagesKept - 1u)
                 {
                     const u32 l = hw::ffs(pagesKept) - 1u;
                     const u32 bits = wave::lane(keep, l);
                     const u32 n = static_cast<u32>(hw::popc(bits));
-                    if(inNeed && myRank >= got && myRank < got + n)
-                    {
-                        const u32 pg = wave::lane(pageL, l);
-                        const u32 cs = wave::lane(csl, l);
-                        out = h.pageData(pg) + static_cast<u64>(wave::selectBit(bits, myRank - got)) * cs;
-                    }
+                    handOut<M>(g, got, n, h.pageData(wave::lane(pageL, l)), bits, wave::lane(csl, l), out);
                     got += n;
                 }
-                return hw::ballot(inNeed && myRank < got);
+                return got;
             }
 
-            //! Claims chunks of a hierarchical chunk size c for the lanes of
-            //! `need` on several pages the scan saw unformatted, in one round
-            //! trip.  Lane l stands for page `pageL` (candidates in `fresh`);
-            //! pages are taken from the rotation point on until their capacity
-            //! covers the group.  Each chosen page's lane formats its page (CAS
-            //! 0 -> c), reserves `take` slots with the 64-bit add (which returns
-            //! the chunk size it was made under) and sets the lowest `take`
-            //! chunk bits in the page's first two segment words, all at once: a
-            //! group takes at most 64 chunks, and every hierarchical chunk size
-            //! has at least one full 32-chunk segment, so two words suffice.
-            //! The reference's tryUsePage (Scatter.hpp:469-517) reserves, then
-            //! re-checks the chunk size, then claims a bit, one round trip each;
-            //! claimOnPage needs two.  Ownership of a chunk is decided by the
-            //! atomicOr's result, as in claimBits: a concurrent claim of the same
-            //! page only costs chunks.  A page that turned out formatted with
-            //! another size, or locked by a reset, gives back what it won.
-            //! Returns the lanes served; `failed` gets the chosen pages that
-            //! yielded no chunk.
-            MALLOCMC_D u64 claimFresh(u32 pageL, u64 fresh, u32 rot, u32 c, u64 need, u64& chosen, u64& failed,
-                                      char*& out) const
+            //! Claims chunks of a hierarchical chunk size c for the waiting
+            //! ranks 0 .. k-1 of `g` on several pages the scan saw unformatted, in
+            //! one round trip.  Lane l stands for page `pageL` (candidates in
+            //! `fresh`); pages are taken from the rotation point on until their
+            //! capacity covers the group.  Each chosen page's lane formats its
+            //! page (CAS 0 -> c), reserves `take` slots with the 64-bit add
+            //! (which returns the chunk size it was made under) and sets the
+            //! lowest `take` chunk bits in the page's first segment words, all
+            //! at once: a page gives a group at most 32 NW chunks (NW = 2 words
+            //! for one request per lane, 4 for more), or its whole capacity
+            //! when that is smaller.  The reference's
+            //! tryUsePage (Scatter.hpp:469-517) reserves, then re-checks the
+            //! chunk size, then claims a bit, one round trip each; claimOnPage
+            //! needs two.  Ownership of a chunk is decided by the atomicOr's
+            //! result, as in claimBits: a concurrent claim of the same page only
+            //! costs chunks.  A page that turned out formatted with another
+            //! size, or locked by a reset, gives back what it won.  Returns how
+            //! many ranks it served, lowest first; `failed` gets the chosen
+            //! pages that yielded no chunk.
+            template<u32 M>
+            MALLOCMC_D u32 claimFresh(u32 pageL, u64 fresh, u32 rot, u32 c, u32 k, const GroupRanks<M>& g,
+                                      u64& chosen, u64& failed, char* (&out)[M]) const
             {
-                const u32 me = hw::laneid();
+                constexpr u32 NW = M >= 2u ? 4u : 2u; // segment words one page's claim may touch
                 const u32 P = h.pagesize;
-                const bool isFresh = ((fresh >> me) & 1u) != 0u;
+                const bool isFresh = ((fresh >> hw::laneid()) & 1u) != 0u;
                 const u32 cap = h.pageCapacity(c);
                 // order among the candidates: from the rotation point, then the rest
                 const u64 hiMask = rot < 64u ? (~0ull << rot) : 0ull;
                 const u64 below = hw::lanemask_lt();
                 const u32 nHi = static_cast<u32>(hw::popc(fresh & hiMask));
-                const bool inHi = isFresh && ((hiMask >> me) & 1u) != 0u;
+                const bool inHi = isFresh && ((hiMask >> hw::laneid()) & 1u) != 0u;
                 const u32 order = inHi ? static_cast<u32>(hw::popc(fresh & hiMask & below))
                                        : nHi + static_cast<u32>(hw::popc(fresh & ~hiMask & below));
-                const u32 k = static_cast<u32>(hw::popc(need));
-                const u32 pos = order * cap;
-                const u32 take = (isFresh && pos < k) ? umin32(cap, k - pos) : 0u;
+                const u32 perPage = umin32(cap, 32u * NW);
+                const u32 pos = order * perPage;
+                const u32 take = (isFresh && pos < k) ? umin32(perPage, k - pos) : 0u;
                 const Layout L = layout(pageL, c);
-                const u32 sp0 = L.full >= 1u ? 32u : L.additional;
-                const u32 sp1 = L.full >= 2u ? 32u : (L.full == 1u ? L.additional : 0u);
-                const u32 t0 = umin32(take, sp0);
-                const u32 t1 = umin32(take - t0, sp1);
-                const u32 m0 = t0 >= 32u ? ~0u : ((1u << t0) - 1u);
-                const u32 m1 = t1 >= 32u ? ~0u : ((1u << t1) - 1u);
+                // the lowest `take` chunks: word w holds 32 (w < full) or
+                // `additional` (w == full) of them
+                u32 m[NW];
+                u32 left = take;
+#pragma unroll
+                for(u32 w = 0; w < NW; ++w)
+                {
+                    const u32 sp = w < L.full ? 32u : (w == L.full ? L.additional : 0u);
+                    const u32 tw = umin32(left, sp);
+                    m[w] = tw >= 32u ? ~0u : ((1u << tw) - 1u);
+                    left -= tw;
+                }
                 chosen = hw::ballot(take != 0u);
-                u32 old = 0u, rcs = 0u, fcs = 1u, ob0 = 0u, ob1 = 0u;
+                u32 old = 0u, rcs = 0u, fcs = 1u;
+                u32 ob[NW];
+#pragma unroll
+                for(u32 w = 0; w < NW; ++w)
+                    ob[w] = 0u;
                 if(take != 0u)
                 {
                     fcs = atom::cas(h.pteChunksize(pageL), 0u, c);
                     const u64 pr = atom::add(h.ptePair(pageL), u64{take} << 32);
-                    ob0 = atom::or_(&L.words[0], m0);
-                    if(m1 != 0u)
-                        ob1 = atom::or_(&L.words[1], m1);
+                    ob[0] = atom::or_(&L.words[0], m[0]);
+#pragma unroll
+                    for(u32 w = 1; w < NW; ++w)
+                        if(m[w] != 0u)
+                            ob[w] = atom::or_(&L.words[w], m[w]);
                     old = static_cast<u32>(pr >> 32);
                     rcs = static_cast<u32>(pr);
                 }
                 const bool formatted = take != 0u && fcs == 0u;
                 const bool locked = h.resetfreedpages && old >= P;
                 const bool usable = take != 0u && rcs == c && !locked;
-                const u32 won0 = take != 0u ? (m0 & ~ob0) : 0u;
-                const u32 won1 = take != 0u ? (m1 & ~ob1) : 0u;
-                const u32 keep0 = usable ? won0 : 0u;
-                const u32 keep1 = usable ? won1 : 0u;
-                const u32 kept = static_cast<u32>(hw::popc(keep0) + hw::popc(keep1));
+                u32 keep[NW];
+                u32 kept = 0u;
+#pragma unroll
+                for(u32 w = 0; w < NW; ++w)
+                {
+                    const u32 won = take != 0u ? (m[w] & ~ob[w]) : 0u;
+                    keep[w] = usable ? won : 0u;
+                    kept += static_cast<u32>(hw::popc(keep[w]));
+                    if(take != 0u && (won & ~keep[w]) != 0u)
+                        atom::and_(&L.words[w], ~(won & ~keep[w]));
+                }
                 if(take != 0u)
                 {
-                    if((won0 & ~keep0) != 0u)
-                        atom::and_(&L.words[0], ~(won0 & ~keep0));
-                    if((won1 & ~keep1) != 0u)
-                        atom::and_(&L.words[1], ~(won1 & ~keep1));
                     if(kept == 0u)
                     {
                         backOff(pageL, take);
@@ -655,26 +765,22 @@ namespace mallocMC
                         atom::sub(h.pteCount(pageL), take - kept); // holds kept chunks: count stays > 0
                 }
                 failed = hw::ballot(take != 0u && kept == 0u);
-                // hand the kept chunks to the group's lanes in rank order
-                const u32 myRank = hw::rankBelow(need);
-                const bool inNeed = ((need >> me) & 1u) != 0u;
+                // hand the kept chunks to the group's ranks in order, word by word
                 u32 got = 0u;
                 for(u64 pagesKept = hw::ballot(kept != 0u); pagesKept != 0u; pagesKept &= pagesKept - 1u)
                 {
                     const u32 l = hw::ffs(pagesKept) - 1u;
-                    const u32 b0 = wave::lane(keep0, l);
-                    const u32 b1 = wave::lane(keep1, l);
-                    const u32 n0 = static_cast<u32>(hw::popc(b0));
-                    const u32 n = n0 + static_cast<u32>(hw::popc(b1));
-                    if(inNeed && myRank >= got && myRank < got + n)
+                    char* const data = h.pageData(wave::lane(pageL, l));
+#pragma unroll
+                    for(u32 w = 0; w < NW; ++w)
                     {
-                        const u32 r = myRank - got;
-                        const u32 idx = r < n0 ? wave::selectBit(b0, r) : 32u + wave::selectBit(b1, r - n0);
-                        out = h.pageData(wave::lane(pageL, l)) + static_cast<u64>(idx) * c;
+                        const u32 bits = wave::lane(keep[w], l);
+                        const u32 n = static_cast<u32>(hw::popc(bits));
+                        handOut<M>(g, got, n, data + static_cast<u64>(32u * w) * c, bits, c, out);
+                        got += n;
                     }
-                    got += n;
                 }
-                return hw::ballot(inNeed && myRank < got);
+                return got;
             }
 
             //! Skips ahead over regions without a candidate page for chunk size
@@ -785,7 +891,13 @@ namespace mallocMC
             //! threshold first) but examines up to 64 pages -- 64/regionsize
             //! regions -- per round trip.
             //! `ffb`: the access-block cursor (_firstfreeblock) as the wave read it.
-            MALLOCMC_D u64 allocGroup(u64 group, u32 c, u32 bmax, u32 maxcs, u32 ffb, char*& out) const
+            //! `need` holds the group's waiting requests (LaneNeed: one per
+            //! lane; RankNeed: M per lane); the returned copy records which
+            //! were served, each with its chunk in `out`.  (Passed by value:
+            //! by reference, the compiler kept the wave malloc's walk in more
+            //! vector registers, 128 instead of 76.)
+            template<u32 M, class Need>
+            MALLOCMC_D Need allocGroup(Need need, u32 c, u32 bmax, u32 maxcs, u32 ffb, char* (&out)[M]) const
             {
                 const u32 me = hw::laneid();
                 const u32 R = h.regionsize;
@@ -822,7 +934,6 @@ namespace mallocMC
                 // regions examined per round trip
                 const u32 W = R >= 64u ? 1u : h.divR.div(64u);
                 const u32 span = R >= 64u ? 64u : W * R;
-                u64 need = group;
                 u32 checklevel = R * 3u / 4u;
                 // chunks above the hierarchy threshold: every fitting page keeps
                 // its chunk bits in the PTE bitmask (candidates have a chunk
@@ -831,9 +942,9 @@ namespace mallocMC
                 // claims on several pages per round trip (claimFlat, claimFresh);
                 // with flat_single every page is claimed on its own (claimOnPage)
                 const bool multi = t.flatFast != 0u;
-                for(u32 finder = 0; finder < 2u && need != 0u; ++finder)
+                for(u32 finder = 0; finder < 2u && need.open(); ++finder)
                 {
-                    for(u32 done = 0; done < numregions && need != 0u; done += W)
+                    for(u32 done = 0; done < numregions && need.open(); done += W)
                     {
                         const u32 nreg = umin32(W, numregions - done);
                         bool windowHadCand = false;
@@ -841,14 +952,14 @@ namespace mallocMC
                         const u32 ri = R >= 64u ? 0u : h.divR.div(me);
                         const u32 region = wrap(region0 + done + ri);
                         const bool inSpan = me < span && ri < nreg;
-                        for(u32 base = 0; base < (R >= 64u ? R : 1u) && need != 0u; base += 64u)
+                        for(u32 base = 0; base < (R >= 64u ? R : 1u) && need.open(); base += 64u)
                         {
                             // Claims race with the other waves that scanned the same
                             // pages: a round whose pages were taken meanwhile stops and
                             // the window is read again, instead of trying the rest of a
                             // stale snapshot page by page (at most kWindowReads reads).
                             u64 raised = 0u; // regions of the window whose fill level this visit raised
-                            for(u32 read = 0; read < kWindowReads && need != 0u; ++read)
+                            for(u32 read = 0; read < kWindowReads && need.open(); ++read)
                             {
                                 bool stale = false;
                                 u32 fill = 0xffffffffu, pcs = 0u, pcount = 0u, pbits = 0u;
@@ -877,7 +988,7 @@ namespace mallocMC
                                 const u64 open = hw::ballot(regionOpen);
                                 windowHadCand = windowHadCand || (usedAll | freeAll) != 0u;
                                 const u32 rounds = R >= 64u ? 1u : nreg;
-                                for(u32 k = 0; k < rounds && need != 0u; ++k)
+                                for(u32 k = 0; k < rounds && need.open(); ++k)
                                 {
                                     const u32 lo = R >= 64u ? 0u : k * R;
                                     const u32 width = R >= 64u ? umin32(64u, R - base) : R;
@@ -890,7 +1001,7 @@ namespace mallocMC
                                     u64 cands = freeAll & sel;
                                     const u32 rotAbs = R >= 64u ? (off0 >= base ? off0 - base : 0u) : off0;
                                     const u32 rot = lo + rotAbs;
-                                    const u64 before = need;
+                                    const u32 before = need.mark();
                                     bool staleK = false;
                                     if(multi)
                                     {
@@ -899,11 +1010,12 @@ namespace mallocMC
                                         {
                                             // rounds over the scanned candidates, used pages
                                             // first; pages a round chose are not offered again
-                                            while(need != 0u && (used | cands) != 0u)
+                                            while(need.open() && (used | cands) != 0u)
                                             {
                                                 u64 chosen = 0u, failed = 0u;
-                                                need &= ~claimFlat(pageL, pcs, pcount, pbits, used, cands, rot, c,
-                                                                   need, chosen, failed, out);
+                                                need.served(wave::uniform(claimFlat<M>(pageL, pcs, pcount, pbits, used, cands, rot, c,
+                                                                         need.remaining(), need.ranks(), chosen,
+                                                                         failed, out)));
                                                 used &= ~chosen;
                                                 cands &= ~chosen;
                                                 staleK = staleK || failed != 0u;
@@ -918,25 +1030,27 @@ namespace mallocMC
                                             // fitting one has room is what fragments the
                                             // reference's heap; then the unformatted pages,
                                             // several per round trip
-                                            while(used != 0u && need != 0u)
+                                            while(used != 0u && need.open())
                                             {
                                                 const u64 hi = rot < 64u ? (used >> rot) << rot : 0u;
                                                 const u32 l = hw::ffs(hi != 0u ? hi : used) - 1u;
                                                 used &= ~(1ull << l);
                                                 const u32 page = regionK * R + (R >= 64u ? base + l : l - lo);
-                                                const u64 served = claimOnPage(page, wave::lane(pcs, l), bmax, maxcs,
-                                                                               need, false, out);
-                                                need &= ~served;
+                                                const u32 served = wave::uniform(claimOnPage<M>(page, wave::lane(pcs, l), bmax,
+                                                                                  maxcs, need.remaining(), need.ranks(),
+                                                                                  false, out));
+                                                need.served(served);
                                                 if(served == 0u)
                                                 {
                                                     staleK = true;
                                                     break;
                                                 }
                                             }
-                                            while(t.freshFast != 0u && need != 0u && cands != 0u && !staleK)
+                                            while(t.freshFast != 0u && need.open() && cands != 0u && !staleK)
                                             {
                                                 u64 chosen = 0u, failed = 0u;
-                                                need &= ~claimFresh(pageL, cands, rot, c, need, chosen, failed, out);
+                                                need.served(wave::uniform(claimFresh<M>(pageL, cands, rot, c, need.remaining(),
+                                                                          need.ranks(), chosen, failed, out)));
                                                 cands &= ~chosen;
                                                 staleK = staleK || failed != 0u;
                                                 if(chosen == 0u || failed != 0u)
@@ -950,7 +1064,7 @@ namespace mallocMC
                                     // one page at a time (flat_single, and the unformatted pages
                                     // of fresh_single), partially used pages of a fitting chunk
                                     // size first, free pages after them
-                                    while((used | cands) != 0u && need != 0u)
+                                    while((used | cands) != 0u && need.open())
                                     {
                                         const u64 set = used != 0u ? used : cands;
                                         const u64 hi = rot < 64u ? (set >> rot) << rot : 0u;
@@ -961,15 +1075,14 @@ namespace mallocMC
                                         const u32 page = regionK * R + (R >= 64u ? base + l : l - lo);
                                         const u32 pc = wave::lane(pcs, l);
                                         // a free page is formatted by the claim itself
-                                        const u64 served = claimOnPage(page, pc == 0u ? c : pc, bmax, maxcs, need,
-                                                                       pc == 0u, out);
-                                        need &= ~served;
+                                        need.served(wave::uniform(claimOnPage<M>(page, pc == 0u ? c : pc, bmax, maxcs,
+                                                                   need.remaining(), need.ranks(), pc == 0u, out)));
                                     }
                                     stale = stale || staleK;
                                     // an unsuccessful visit raises the region's fill level
                                     // once (Scatter.hpp:612-617); a visit cut short by a
                                     // stale snapshot is not one yet
-                                    if(need == before && !staleK)
+                                    if(need.mark() == before && !staleK)
                                     {
                                         if(((raised >> k) & 1u) == 0u && fillK + 1u <= R && me == 0u)
                                             atom::cas(&h.regions[regionK], fillK, fillK + 1u);
@@ -996,12 +1109,12 @@ namespace mallocMC
                             atom::store(&h.scalars->firstfreeblock, abid);
                         // a window with no candidate page at all: skip ahead over
                         // the regions that have none either, 256 pages per round trip
-                        if(!windowHadCand && need != 0u && R < 64u && t.fastForward != 0u && done + W < numregions)
+                        if(!windowHadCand && need.open() && R < 64u && t.fastForward != 0u && done + W < numregions)
                             done += fastForward(region0, done + W, numregions, checklevel, bmax, maxcs, wrap);
                     }
                     checklevel = R + 1u;
                 }
-                return group & ~need;
+                return need;
             }
 
             //! Marks pages [sp, sp + k) as one run of `bytes` (chunk size =
@@ -1238,10 +1351,10 @@ namespace mallocMC
                     const u32 bmax = wave::max(member ? b : 0u);
                     const u32 bmin = wave::min(member ? b : 0xffffffffu);
                     const u32 maxcs = umin32(h.pagesize, umax32(h.wastefactor * bmin, h.paddedMinChunk));
-                    char* mine = nullptr;
-                    const u64 served = allocGroup(group, c, bmax, maxcs, ffb, mine);
-                    if(member && ((served >> me) & 1u))
-                        result = mine;
+                    char* mine[1] = {nullptr};
+                    const LaneNeed need = allocGroup<1>(LaneNeed{group}, c, bmax, maxcs, ffb, mine);
+                    if(member && ((need.bits >> me) & 1u) == 0u)
+                        result = mine[0];
                     // lanes of the group not served: the heap has no room for them
                     pending &= ~group;
                 }
@@ -1249,6 +1362,142 @@ namespace mallocMC
                 if(isOOM(r, b))
                     r = nullptr;
                 return dist.distribute(r);
+            }
+
+            //! Wave malloc of M requests per lane: slot j of lane l is the
+            //! wave's request 64 j + l.  For callers that hand a wave more
+            //! requests than it has lanes (the batch kernels): a size-class
+            //! group then spans up to 64 M requests, served by the same scans
+            //! and claims as malloc(), so the work a group pays once (its scan
+            //! windows, claim round trips and wave reductions) is spread over
+            //! more requests.  Groups are capped at aggregation x M requests.
+            //! There is no distribution policy: XMallocSIMD coalesces the 64
+            //! lanes of one call and keeps malloc().
+            template<u32 M>
+            MALLOCMC_D void mallocMany(const u32 (&bytesIn)[M], void* (&res)[M]) const
+            {
+                const u32 me = hw::laneid();
+                u32 ffb = 0u;
+                if(me == 0u)
+                    ffb = atom::load(&h.scalars->firstfreeblock);
+                u32 b[M];
+                char* result[M];
+                bool want[M];
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    b[j] = h.pad(bytesIn[j]);
+                    result[j] = nullptr;
+                    want[j] = b[j] != 0u;
+                }
+                // requests above a page: one wave-wide run claim each, as in
+                // malloc(); a run of k pages that was not found ends the
+                // search for runs of k or more
+                u32 failedPages = 0xffffffffu;
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    for(u64 big = hw::ballot(want[j] && b[j] > h.pagesize); big != 0u; big &= big - 1u)
+                    {
+                        const u32 l = hw::ffs(big) - 1u;
+                        const u32 bl = wave::lane(b[j], l);
+                        const u32 kl = h.divP.div(bl + h.pagesize - 1u);
+                        char* r = nullptr;
+                        if(kl < failedPages)
+                        {
+                            r = allocRun(bl);
+                            if(r == nullptr)
+                                failedPages = kl;
+                        }
+                        if(me == l)
+                            result[j] = r;
+                    }
+                    if(b[j] > h.pagesize)
+                        want[j] = false;
+                }
+                u64 pending[M];
+                u32 cls[M];
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    pending[j] = hw::ballot(want[j]);
+                    cls[j] = want[j] ? sizeClass(h, b[j], t.classSplit) : 0xffffffffu;
+                }
+                ffb = wave::lane(ffb, 0u);
+                const u32 cap = t.aggregation * M;
+                for(;;)
+                {
+                    // the leader: the lowest pending request in (slot, lane) order
+                    u32 lj = M;
+#pragma unroll
+                    for(u32 j = M; j-- > 0u;)
+                        if(pending[j] != 0u)
+                            lj = j;
+                    if(lj == M)
+                        break;
+                    u64 lpend = 0u;
+                    u32 lc = 0u;
+#pragma unroll
+                    for(u32 j = 0; j < M; ++j)
+                        if(j == lj)
+                        {
+                            lpend = pending[j];
+                            lc = cls[j];
+                        }
+                    const u32 lcls = wave::lane(lc, hw::ffs(lpend) - 1u);
+                    RankNeed<M> need;
+                    u64 group[M];
+                    u32 K = 0u;
+#pragma unroll
+                    for(u32 j = 0; j < M; ++j)
+                    {
+                        const bool in = ((pending[j] >> me) & 1u) != 0u && cls[j] == lcls;
+                        group[j] = hw::ballot(in);
+                        need.rank.rank[j] = in ? K + hw::rankBelow(group[j]) : kNoRank;
+                        K += static_cast<u32>(hw::popc(group[j]));
+                    }
+                    if(K > cap)
+                    {
+                        // the lowest `cap` ranks form the group; the rest stay pending
+#pragma unroll
+                        for(u32 j = 0; j < M; ++j)
+                        {
+                            if(need.rank.rank[j] >= cap)
+                                need.rank.rank[j] = kNoRank;
+                            group[j] = hw::ballot(need.rank.rank[j] != kNoRank);
+                        }
+                        K = cap;
+                    }
+                    need.K = K;
+                    need.got = 0u;
+                    u32 cMax = 0u, bMax = 0u, bMin = 0xffffffffu;
+#pragma unroll
+                    for(u32 j = 0; j < M; ++j)
+                        if(need.rank.rank[j] != kNoRank)
+                        {
+                            cMax = umax32(cMax, umax32(b[j], h.paddedMinChunk));
+                            bMax = umax32(bMax, b[j]);
+                            bMin = umin32(bMin, b[j]);
+                        }
+                    const u32 c = wave::max(cMax);
+                    const u32 bmax = wave::max(bMax);
+                    const u32 bmin = wave::min(bMin);
+                    const u32 maxcs = umin32(h.pagesize, umax32(h.wastefactor * bmin, h.paddedMinChunk));
+                    // chunks go straight to `result`: only served ranks are written
+                    need = allocGroup<M>(need, c, bmax, maxcs, ffb, result);
+                    // requests of the group not served: the heap has no room for them
+#pragma unroll
+                    for(u32 j = 0; j < M; ++j)
+                        pending[j] &= ~group[j];
+                }
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    void* r = result[j];
+                    if(isOOM(r, b[j]))
+                        r = nullptr;
+                    res[j] = r;
+                }
             }
 
             //! Wave free: every lane passes its pointer or nullptr.  Lanes are
@@ -1331,32 +1580,220 @@ namespace mallocMC
                 {
                     atom::and_(h.pteBitmask(page), ~pageBits);
                     old = atom::sub(h.pteCount(page), pageK);
-                    // region reset when the count passes pagesize/2/c (Scatter.hpp:727-735),
-                    // at least when the page empties: for c > pagesize/2 the reference's
-                    // threshold is 0 and never fires, so failed scans ratchet every
-                    // region to "full" and each later request scans the whole heap.
-                    // Test first so hot words are only written when they change.
-                    const u32 thr = umax32(h.pdiv(P / 2u, cs), 1u);
-                    if(old >= thr && old - pageK < thr)
+                    afterRelease(page, cs, old, pageK);
+                }
+            }
+
+            //! One lane, after its group returned k chunks of `page` (chunk
+            //! size cs) and the count read `old` before the decrement.
+            //! Region reset when the count passes pagesize/2/c
+            //! (Scatter.hpp:727-735), at least when the page empties: for
+            //! c > pagesize/2 the reference's threshold is 0 and never fires,
+            //! so failed scans ratchet every region to "full" and each later
+            //! request scans the whole heap.  Hot words are tested first, so
+            //! they are only written when they change.
+            //! Then D4 for k chunks: the group that empties its page resets
+            //! it.  The reference locks the count, clears the masks
+            //! (Page::init), drops the chunk size, fences and unlocks
+            //! (Scatter.hpp:686-716).  The side masks hold no bit of a freed
+            //! chunk (each free clears its bits; a clear still in flight only
+            //! keeps a chunk that no one owns out of use until it lands), and
+            //! chunk size and count share one 64-bit word, so one CAS
+            //! (cs, 0) -> (0, 0) resets the page if no one reserved it
+            //! meanwhile.
+            MALLOCMC_D void afterRelease(u32 page, u32 cs, u32 old, u32 k) const
+            {
+                const u32 thr = umax32(h.pdiv(h.pagesize / 2u, cs), 1u);
+                if(old >= thr && old - k < thr)
+                {
+                    u32* region = &h.regions[h.divR.div(page)];
+                    if(atom::load(region) != 0u)
+                        atom::exch(region, 0u);
+                    const u32 block = h.divPpb.div(page);
+                    if(block < atom::load(&h.scalars->firstfreeblock))
+                        atom::min(&h.scalars->firstfreeblock, block);
+                }
+                if(h.resetfreedpages && old == k)
+                    atom::cas(h.ptePair(page), u64{cs}, u64{0});
+            }
+
+            //! Wave free of M pointers per lane (slot j of lane l is the wave's
+            //! request 64 j + l), grouped by page and segment over all 64 M
+            //! slots, for callers that hand a wave more frees than it has
+            //! lanes (the batch kernels): a page's frees meet in fewer groups,
+            //! and a group's atomics are issued once for all its chunks.
+            template<u32 M>
+            MALLOCMC_D void freeMany(void* const (&pv)[M]) const
+            {
+                const u32 me = hw::laneid();
+                const u32 P = h.pagesize;
+                u32 page[M], cs[M], idx[M];
+                bool simple[M];
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    page[j] = 0u;
+                    cs[j] = 0u;
+                    if(pv[j] != nullptr)
                     {
-                        u32* region = &h.regions[h.divR.div(page)];
-                        if(atom::load(region) != 0u)
-                            atom::exch(region, 0u);
-                        const u32 block = h.divPpb.div(page);
-                        if(block < atom::load(&h.scalars->firstfreeblock))
-                            atom::min(&h.scalars->firstfreeblock, block);
+                        page[j] = h.pageOf(static_cast<u64>(static_cast<char*>(pv[j]) - h.pages));
+                        cs[j] = atom::load(h.pteChunksize(page[j]));
                     }
                 }
-                // ---- D4 for k chunks: the group that empties its page resets it.
-                // The reference locks the count, clears the masks (Page::init),
-                // drops the chunk size, fences and unlocks (Scatter.hpp:686-716).
-                // The side masks hold no bit of a freed chunk (each free clears
-                // its bits; a clear still in flight only keeps a chunk that no
-                // one owns out of use until it lands), and chunk size and count
-                // share one 64-bit word, so one CAS (cs, 0) -> (0, 0) resets the
-                // page if no one reserved it meanwhile.
-                if(h.resetfreedpages && pageLeader && old == pageK)
-                    atom::cas(h.ptePair(page), u64{cs}, u64{0});
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    char* p = static_cast<char*>(pv[j]);
+                    simple[j] = false;
+                    idx[j] = 0u;
+                    if(p != nullptr)
+                    {
+                        const u32 inpage = static_cast<u32>(p - h.pageData(page[j]));
+                        const bool chunked = cs[j] != 0u
+                            && (cs[j] < P || (cs[j] == P && (h.flags & kFixPagesizeLeak)));
+                        const u32 q = chunked ? h.pdiv(inpage, cs[j]) : 0u;
+                        simple[j] = chunked && q * cs[j] == inpage;
+                        if(!simple[j])
+                            h.destroy<WaveIds>(p); // multi-page, XMallocSIMD slice, stale pointer
+                        idx[j] = simple[j] ? q : 0u;
+                    }
+                }
+
+                // ---- grouping (registers only); the leader of a group is its
+                // lowest slot in (slot, lane) order
+                bool pageLeader[M], segLeader[M];
+                u32 pageK[M], pageBits[M], segBits[M];
+                u64 pending[M];
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    pageLeader[j] = false;
+                    segLeader[j] = false;
+                    pageK[j] = 0u;
+                    pageBits[j] = 0u;
+                    segBits[j] = 0u;
+                    pending[j] = hw::ballot(simple[j]);
+                }
+                for(;;)
+                {
+                    u32 lj = M;
+#pragma unroll
+                    for(u32 j = M; j-- > 0u;)
+                        if(pending[j] != 0u)
+                            lj = j;
+                    if(lj == M)
+                        break;
+                    u64 lpend = 0u;
+                    u32 lp = 0u, lcs = 0u;
+#pragma unroll
+                    for(u32 j = 0; j < M; ++j)
+                        if(j == lj)
+                        {
+                            lpend = pending[j];
+                            lp = page[j];
+                            lcs = cs[j];
+                        }
+                    const u32 leader = hw::ffs(lpend) - 1u;
+                    const u32 lpage = wave::lane(lp, leader);
+                    // every member shares the page, hence its chunk size
+                    const bool hier = wave::lane(lcs, leader) <= h.hierarchyThreshold;
+                    u64 group[M];
+                    u32 k = 0u, bitsIn = 0u;
+#pragma unroll
+                    for(u32 j = 0; j < M; ++j)
+                    {
+                        const bool member = ((pending[j] >> me) & 1u) != 0u && page[j] == lpage;
+                        group[j] = hw::ballot(member);
+                        k += static_cast<u32>(hw::popc(group[j]));
+                        // PTE bitmask bits to clear: chunk bits (flat) or segment hints (hierarchical)
+                        if(member)
+                            bitsIn |= hier ? (1u << (idx[j] / 32u)) : (1u << idx[j]);
+                    }
+                    const u32 bits = wave::bor(bitsIn);
+#pragma unroll
+                    for(u32 j = 0; j < M; ++j)
+                        if(j == lj && me == leader)
+                        {
+                            pageLeader[j] = true;
+                            pageK[j] = k;
+                            pageBits[j] = bits;
+                        }
+                    if(hier)
+                    {
+                        u64 segPending[M];
+#pragma unroll
+                        for(u32 j = 0; j < M; ++j)
+                            segPending[j] = group[j];
+                        for(;;)
+                        {
+                            u32 sj = M;
+#pragma unroll
+                            for(u32 j = M; j-- > 0u;)
+                                if(segPending[j] != 0u)
+                                    sj = j;
+                            if(sj == M)
+                                break;
+                            u64 spend = 0u;
+                            u32 sidx = 0u;
+#pragma unroll
+                            for(u32 j = 0; j < M; ++j)
+                                if(j == sj)
+                                {
+                                    spend = segPending[j];
+                                    sidx = idx[j];
+                                }
+                            const u32 sl = hw::ffs(spend) - 1u;
+                            const u32 seg = wave::lane(sidx, sl) / 32u;
+                            u32 segIn = 0u;
+#pragma unroll
+                            for(u32 j = 0; j < M; ++j)
+                            {
+                                const bool inSeg = ((segPending[j] >> me) & 1u) != 0u && idx[j] / 32u == seg;
+                                if(inSeg)
+                                    segIn |= 1u << (idx[j] % 32u);
+                                segPending[j] &= ~hw::ballot(inSeg);
+                            }
+                            const u32 b = wave::bor(segIn);
+#pragma unroll
+                            for(u32 j = 0; j < M; ++j)
+                                if(j == sj && me == sl)
+                                {
+                                    segLeader[j] = true;
+                                    segBits[j] = b;
+                                }
+                        }
+                    }
+#pragma unroll
+                    for(u32 j = 0; j < M; ++j)
+                        pending[j] &= ~group[j];
+                }
+
+                // ---- clear the chunk bits and return the counts: every group at once
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                    if(segLeader[j])
+                    {
+                        const u32 segmentsize = cs[j] * 32u + 4u;
+                        const u32 full = umin32(32u, h.pdiv(P, segmentsize));
+                        const u32 additional = h.calcAdditional(full, segmentsize, cs[j]);
+                        u32* masks = h.onPageMasks(page[j], full + (additional > 0u ? 1u : 0u));
+                        atom::and_(&masks[idx[j] / 32u], ~segBits[j]);
+                    }
+                u32 old[M];
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    old[j] = 0u;
+                    if(pageLeader[j])
+                    {
+                        atom::and_(h.pteBitmask(page[j]), ~pageBits[j]);
+                        old[j] = atom::sub(h.pteCount(page[j]), pageK[j]);
+                    }
+                }
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                    if(pageLeader[j])
+                        afterRelease(page[j], cs[j], old[j], pageK[j]);
             }
         };
     } // namespace detail

@@ -37,6 +37,10 @@ typedef struct mmc_tuning {
     uint32_t fresh_single;  /* 1: claim the unformatted pages of chunk sizes with a segment hierarchy
                                one at a time, two round trips each (0, the default: several per round
                                trip; flat_single = 1 implies this) */
+    uint32_t lane_requests; /* requests per lane of the non-persistent, partitioned wave malloc on a
+                               heap without XMallocSIMD: 1, 2 or 4 (0 = 1).  With m > 1 a block serves
+                               m x block_threads requests, a size-class group spans up to
+                               m x aggregation of them, and the wave free serves two per lane */
 } mmc_tuning;
 
 int mmc_default_tuning(mmc_tuning* t);

@@ -60,6 +60,7 @@ class MmcTuning(ctypes.Structure):
         ("no_partition", ctypes.c_uint32),
         ("block_threads", ctypes.c_uint32),
         ("fresh_single", ctypes.c_uint32),
+        ("lane_requests", ctypes.c_uint32),
     ]
 
 
@@ -74,7 +75,9 @@ class Tuning:
     malloc and free, of the non-persistent multi-heap step and of the fill
     (block_threads: 256, 512 or 1024; 0 = 1024), and whether unformatted
     pages of hierarchical chunk sizes are claimed one at a time
-    (fresh_single)."""
+    (fresh_single), and the requests each lane of the partitioned wave
+    malloc serves (lane_requests: 1, 2 or 4; 0 = 1; the free then serves
+    two per lane)."""
     aggregation: int = 64
     xcd_slices: int = 0
     persistent: bool = False
@@ -85,11 +88,13 @@ class Tuning:
     no_partition: bool = False
     block_threads: int = 0
     fresh_single: bool = False
+    lane_requests: int = 0
 
     def to_c(self) -> MmcTuning:
         return MmcTuning(self.aggregation, self.xcd_slices, int(bool(self.persistent)), self.waves_per_cu,
                          self.class_split, int(bool(self.flat_single)), int(bool(self.no_skip)),
-                         int(bool(self.no_partition)), self.block_threads, int(bool(self.fresh_single)))
+                         int(bool(self.no_partition)), self.block_threads, int(bool(self.fresh_single)),
+                         self.lane_requests)
 
 
 class MmcStepDesc(ctypes.Structure):

@@ -77,29 +77,35 @@ namespace
         return 223u - (8u * (e - 2u) + ((x >> (e - 3u)) & 7u));  // 0..215
     }
 
-    //! Reorders the B values of a block (B = 256, 512 or 1024 threads) by an
-    //! 8-bit key (LDS counting sort, unstable).  Returns the value this
-    //! thread receives and the block slot it came from.  Three barriers:
-    //! after clearing the bucket counts, after counting, after scattering.
-    //! Between the last two, every wave scans the 256 counts itself and
-    //! writes the same bucket starts to LDS (a wave reads only after its own
-    //! writes, and every wave writes the same values), instead of four waves
-    //! scanning and two more barriers to share their totals.
-    template<class T, u32 B>
-    __device__ void partitionByKey(u32 key, T value, T& outValue, u32& outSlot)
+    //! Reorders the B x M values of a block (B = 256, 512 or 1024 threads, M
+    //! values per thread) by an 8-bit key (LDS counting sort, unstable).
+    //! Thread t holds values t + j B (j < M) on entry; lane l of wave w
+    //! receives, as its value j, the value at sorted position 64 (M w + j) + l,
+    //! with the block slot it came from.  Three barriers: after clearing the
+    //! bucket counts, after counting, after scattering.  Between the last
+    //! two, every wave scans the 256 counts itself and writes the same bucket
+    //! starts to LDS (a wave reads only after its own writes, and every wave
+    //! writes the same values), instead of four waves scanning and two more
+    //! barriers to share their totals.
+    template<class T, u32 B, u32 M>
+    __device__ void partitionByKeyMany(const u32 (&key)[M], const T (&value)[M], T (&outValue)[M],
+                                       u32 (&outSlot)[M])
     {
         static_assert(B % 256u == 0u && B <= 1024u, "a block of 256, 512 or 1024 threads");
         static_assert(kSizeBuckets == 4u * 64u, "four buckets per lane in the scan");
         __shared__ u32 count[kSizeBuckets];
         __shared__ u32 start[kSizeBuckets];
-        __shared__ T sortedValue[B];
-        __shared__ u32 sortedSlot[B];
+        __shared__ T sortedValue[B * M];
+        __shared__ u32 sortedSlot[B * M];
         const u32 tid = threadIdx.x;
         const u32 me = hw::laneid();
         if(tid < kSizeBuckets)
             count[tid] = 0u;
         __syncthreads();
-        const u32 rank = atomicAdd(&count[key], 1u);
+        u32 rank[M];
+#pragma unroll
+        for(u32 j = 0; j < M; ++j)
+            rank[j] = atomicAdd(&count[key[j]], 1u);
         __syncthreads();
         // lane l: buckets 4l .. 4l+3, then an inclusive scan of the lane sums
         const u32 c0 = count[4u * me], c1 = count[4u * me + 1u], c2 = count[4u * me + 2u], c3 = count[4u * me + 3u];
@@ -120,12 +126,34 @@ namespace
         // wave complete in order; the fence keeps the compiler from moving
         // the reads up)
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const u32 pos = start[key] + rank;
-        sortedValue[pos] = value;
-        sortedSlot[pos] = tid;
+#pragma unroll
+        for(u32 j = 0; j < M; ++j)
+        {
+            const u32 pos = start[key[j]] + rank[j];
+            sortedValue[pos] = value[j];
+            sortedSlot[pos] = tid + j * B;
+        }
         __syncthreads();
-        outValue = sortedValue[tid];
-        outSlot = sortedSlot[tid];
+        const u32 first = (tid - me) * M + me; // 64 M w + l
+#pragma unroll
+        for(u32 j = 0; j < M; ++j)
+        {
+            outValue[j] = sortedValue[first + 64u * j];
+            outSlot[j] = sortedSlot[first + 64u * j];
+        }
+    }
+
+    //! One value per thread: thread t receives the value at sorted position t.
+    template<class T, u32 B>
+    __device__ void partitionByKey(u32 key, T value, T& outValue, u32& outSlot)
+    {
+        const u32 k[1] = {key};
+        const T v[1] = {value};
+        T ov[1];
+        u32 os[1];
+        partitionByKeyMany<T, B, 1>(k, v, ov, os);
+        outValue = ov[0];
+        outSlot = os[0];
     }
 
     //! The B requests of a block are served by its B/64 waves; the whole
@@ -151,6 +179,33 @@ namespace
             offsets[base + slot] = h.offsetOf(p);
     }
 
+    //! M requests per lane (M = 2): the block's B x M requests are reordered
+    //! by size bucket, and each wave serves 64 M of them through
+    //! WaveAlloc::mallocMany, so that a size-class group spans up to 64 M
+    //! requests and its scans and claims are paid once for all of them.
+    template<u32 B, u32 M>
+    __global__ __launch_bounds__(B) void waveMallocManyKernel(ScatterHeap h, WaveTuning t, const u32* sizes,
+                                                              u64* offsets, u64 n)
+    {
+        const u64 base = static_cast<u64>(blockIdx.x) * (B * M);
+        u32 key[M], bytes[M], slot[M];
+#pragma unroll
+        for(u32 j = 0; j < M; ++j)
+        {
+            const u64 i = base + threadIdx.x + j * B;
+            bytes[j] = i < n ? sizes[i] : 0u;
+            key[j] = sizeBucket(bytes[j], h.pagesize);
+        }
+        partitionByKeyMany<u32, B, M>(key, bytes, bytes, slot);
+        const WaveAlloc wa{h, t};
+        void* p[M];
+        wa.mallocMany<M>(bytes, p);
+#pragma unroll
+        for(u32 j = 0; j < M; ++j)
+            if(base + slot[j] < n)
+                offsets[base + slot[j]] = h.offsetOf(p[j]);
+    }
+
     //! With `partition` the block's frees are first reordered by page index
     //! mod 256, so that the frees of one page gather in one wave and share
     //! its atomics.
@@ -168,6 +223,31 @@ namespace
         }
         const WaveAlloc wa{h, t};
         wa.free(off == kNullOffset ? nullptr : h.pages + off);
+    }
+
+    //! M frees per lane: the block's B x M frees are reordered by page
+    //! index mod 256, and each wave frees 64 M of them through
+    //! WaveAlloc::freeMany, so that more of a page's frees meet in one group.
+    template<u32 B, u32 M>
+    __global__ __launch_bounds__(B) void waveFreeManyKernel(ScatterHeap h, WaveTuning t, const u64* offsets, u64 n)
+    {
+        const u64 base = static_cast<u64>(blockIdx.x) * (B * M);
+        u32 key[M], slot[M];
+        u64 off[M];
+#pragma unroll
+        for(u32 j = 0; j < M; ++j)
+        {
+            const u64 i = base + threadIdx.x + j * B;
+            off[j] = i < n ? offsets[i] : kNullOffset;
+            key[j] = off[j] == kNullOffset ? 255u : (h.pageOf(off[j]) & 255u);
+        }
+        partitionByKeyMany<u64, B, M>(key, off, off, slot);
+        const WaveAlloc wa{h, t};
+        void* p[M];
+#pragma unroll
+        for(u32 j = 0; j < M; ++j)
+            p[j] = off[j] == kNullOffset ? nullptr : h.pages + off[j];
+        wa.freeMany<M>(p);
     }
 
     //! Work-queue helpers shared by the persistent kernels: cursor[0] hands out
@@ -758,6 +838,7 @@ int mmc_default_tuning(mmc_tuning* t)
     t->no_partition = 0u;
     t->block_threads = 0u;
     t->fresh_single = 0u;
+    t->lane_requests = 0u;
     return MMC_OK;
 }
 
@@ -789,7 +870,34 @@ int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_o
         const u32 part = partitionOf(tuning);
         const u32 bs = blockOf(tuning);
         const unsigned blocks = mmc::gridFor(n, bs);
-        if(xm)
+        if(!xm && part != 0u && tuning && tuning->lane_requests >= 4u)
+        {
+            const unsigned blocks4 = mmc::gridFor(n, 4u * bs);
+            if(bs == 1024u)
+                hipLaunchKernelGGL((waveMallocManyKernel<1024, 4>), dim3(blocks4), dim3(1024), 0, s, heap->view, t,
+                                   d_sizes, d_offsets, n);
+            else if(bs == 512u)
+                hipLaunchKernelGGL((waveMallocManyKernel<512, 4>), dim3(blocks4), dim3(512), 0, s, heap->view, t,
+                                   d_sizes, d_offsets, n);
+            else
+                hipLaunchKernelGGL((waveMallocManyKernel<256, 4>), dim3(blocks4), dim3(256), 0, s, heap->view, t,
+                                   d_sizes, d_offsets, n);
+        }
+        else if(!xm && part != 0u && tuning && tuning->lane_requests >= 2u)
+        {
+            // two requests per lane: a block serves 2 x bs of them
+            const unsigned blocks2 = mmc::gridFor(n, 2u * bs);
+            if(bs == 1024u)
+                hipLaunchKernelGGL((waveMallocManyKernel<1024, 2>), dim3(blocks2), dim3(1024), 0, s, heap->view, t,
+                                   d_sizes, d_offsets, n);
+            else if(bs == 512u)
+                hipLaunchKernelGGL((waveMallocManyKernel<512, 2>), dim3(blocks2), dim3(512), 0, s, heap->view, t,
+                                   d_sizes, d_offsets, n);
+            else
+                hipLaunchKernelGGL((waveMallocManyKernel<256, 2>), dim3(blocks2), dim3(256), 0, s, heap->view, t,
+                                   d_sizes, d_offsets, n);
+        }
+        else if(xm)
         {
             if(bs == 1024u)
                 hipLaunchKernelGGL((waveMallocKernel<XMallocSIMD64, 1024>), dim3(blocks), dim3(1024), 0, s,
@@ -833,7 +941,19 @@ int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, c
     else
     {
         const u32 bs = blockOf(tuning);
-        if(bs == 1024u)
+        // with lane_requests 2 or 4, two frees per lane (four measured
+        // slower: a launch of one workgroup per CU waits for its slowest wave)
+        const bool two = tuning && partitionOf(tuning) && tuning->lane_requests >= 2u;
+        if(two && bs == 1024u)
+            hipLaunchKernelGGL((waveFreeManyKernel<1024, 2>), dim3(mmc::gridFor(n, 2048)), dim3(1024), 0, s,
+                               heap->view, t, d_offsets, n);
+        else if(two && bs == 512u)
+            hipLaunchKernelGGL((waveFreeManyKernel<512, 2>), dim3(mmc::gridFor(n, 1024)), dim3(512), 0, s,
+                               heap->view, t, d_offsets, n);
+        else if(two)
+            hipLaunchKernelGGL((waveFreeManyKernel<256, 2>), dim3(mmc::gridFor(n, 512)), dim3(256), 0, s,
+                               heap->view, t, d_offsets, n);
+        else if(bs == 1024u)
             hipLaunchKernelGGL(waveFreeKernel<1024>, dim3(mmc::gridFor(n, 1024)), dim3(1024), 0, s, heap->view, t,
                                d_offsets, n, partitionOf(tuning));
         else if(bs == 512u)

@@ -36,6 +36,10 @@ TUNINGS = [
     dict(aggregation=64, xcd_slices=8, persistent=False, block_threads=256),
     dict(aggregation=32, xcd_slices=0, persistent=False, block_threads=512),
     dict(aggregation=64, xcd_slices=0, persistent=False, fresh_single=True),
+    dict(aggregation=64, xcd_slices=8, persistent=False, lane_requests=2),
+    dict(aggregation=64, xcd_slices=0, persistent=False, lane_requests=4),
+    dict(aggregation=16, xcd_slices=0, persistent=False, lane_requests=2, block_threads=512),
+    dict(aggregation=64, xcd_slices=8, persistent=False, lane_requests=4, block_threads=256, fresh_single=True),
 ]
 
 
@@ -126,8 +130,10 @@ def test_wave_xmallocsimd_and_large(mmc, cuda):
     a.close()
 
 
-@pytest.mark.parametrize("no_partition,block_threads", [(False, 0), (True, 0), (False, 256), (False, 512)])
-def test_wave_batch_output_slots(mmc, cuda, no_partition, block_threads):
+@pytest.mark.parametrize("no_partition,block_threads,lane_requests",
+                         [(False, 0, 0), (True, 0, 0), (False, 256, 0), (False, 512, 0), (False, 0, 2),
+                          (False, 0, 4), (False, 256, 2), (False, 512, 4)])
+def test_wave_batch_output_slots(mmc, cuda, no_partition, block_threads, lane_requests):
     """Each request's offset lands in its own output slot when a block
     reorders its requests by size: checked against the heap's own metadata
     (every live allocation is marked, fits its chunk and is not shared, and
@@ -135,7 +141,7 @@ def test_wave_batch_output_slots(mmc, cuda, no_partition, block_threads):
     empty and multi-page requests."""
     from mallocmc_amd import snapshot
     torch = cuda
-    t = mmc.Tuning(no_partition=no_partition, block_threads=block_threads)
+    t = mmc.Tuning(no_partition=no_partition, block_threads=block_threads, lane_requests=lane_requests)
     cfg = mmc.ScatterConfig(pagesize=16384, accessblocksize=256 * MiB, regionsize=16, resetfreedpages=True)
     a = mmc.ScatterAllocator(cfg, 256 * MiB)
     g = torch.Generator().manual_seed(3)
@@ -144,7 +150,8 @@ def test_wave_batch_output_slots(mmc, cuda, no_partition, block_threads):
     sizes[torch.randint(0, n, (500,), generator=g)] = 0
     sizes[torch.randint(0, n, (60,), generator=g)] = 3 * 16384 + 100
     sizes = sizes.to(torch.int32).cuda()
-    print(f"wave output slots no_partition={no_partition} block_threads={block_threads} n={n}", flush=True)
+    print(f"wave output slots no_partition={no_partition} block_threads={block_threads} "
+          f"lane_requests={lane_requests} n={n}", flush=True)
     offs = a.wave_malloc_batch(sizes, tuning=t)
     o = offs.cpu().numpy()
     z = sizes.cpu().numpy() == 0
