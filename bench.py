@@ -56,7 +56,8 @@ COUNTERS_PROFILE = os.path.join(HERE, "profiles", "batched_mixed1M_counters.json
 
 def stored_point():
     from mallocmc_amd import autotune
-    return autotune.TunePoint(pagesize=65536, regionsize=16, aggregation=64, persistent=False, xcd_slices=8)
+    return autotune.TunePoint(pagesize=65536, regionsize=32, aggregation=64, persistent=False, xcd_slices=0,
+                              lane_requests=2)
 
 
 def log(*a):
@@ -250,16 +251,17 @@ def atomic_roofline_of(point, mal_avg_ms, ceiling):
     run = counter_run(point)
     if run is None:
         return {"bound": "atomic", "note": f"no counter pass at {point_key(point)} in {PROFILE_NAME}"}
-    per_launch = run["waveMallocKernel"]["TCC_ATOMIC_sum"]
+    mk, fk = kernel_names(point)
+    per_launch = run[mk]["TCC_ATOMIC_sum"]
     per_op = run["atomicsKernel"]["TCC_ATOMIC_per_lane_op"]
     achieved = per_launch / (mal_avg_ms / 1e3)
     peak = ceiling * per_op
     return {
         "bound": "atomic", "achieved": achieved, "peak": peak, "unit": "L2 atomic requests/s",
-        "frac": achieved / peak, "kernel": "waveMallocKernel",
+        "frac": achieved / peak, "kernel": mk,
         "atomics_per_launch": per_launch, "atomics_per_request": per_launch / N_LANES,
-        "atomic_instructions_per_launch": run["waveMallocKernel"].get("TA_FLAT_ATOMIC_WAVEFRONTS_sum"),
-        "free_kernel_atomics_per_launch": run.get("waveFreeKernel", {}).get("TCC_ATOMIC_sum"),
+        "atomic_instructions_per_launch": run[mk].get("TA_FLAT_ATOMIC_WAVEFRONTS_sum"),
+        "free_kernel_atomics_per_launch": run.get(fk, {}).get("TCC_ATOMIC_sum"),
         "profile": PROFILE_NAME, "profile_point": point_key(point),
     }
 
@@ -277,7 +279,15 @@ def counter_run(point):
 
 def point_key(point):
     """Key of a tuning point in the counter profiles."""
-    return f"P{point.pagesize}_R{point.regionsize}_A{point.aggregation}_X{point.xcd_slices}"
+    key = f"P{point.pagesize}_R{point.regionsize}_A{point.aggregation}_X{point.xcd_slices}"
+    return key if point.lane_requests <= 1 else f"{key}_L{point.lane_requests}"
+
+
+def kernel_names(point):
+    """(malloc, free) kernels a one-heap batch step launches at `point`."""
+    if point.lane_requests > 1:
+        return "waveMallocManyKernel", "waveFreeManyKernel"
+    return "waveMallocKernel", "waveFreeKernel"
 
 
 def main():
@@ -296,6 +306,8 @@ def main():
                     help="skip the batched form at the plain form's geometry (profiling runs)")
     ap.add_argument("--xcd-slices", type=int, default=None,
                     help="with --no-tune: XCD slices of the stored point (for counter comparisons)")
+    ap.add_argument("--lane-requests", type=int, default=None,
+                    help="with --no-tune: requests per lane of the stored point (for counter comparisons)")
     ap.add_argument("--regionsize", type=int, default=None,
                     help="with --no-tune: regionsize of the stored point (for counter comparisons)")
     args = ap.parse_args()
@@ -325,7 +337,8 @@ def main():
     # pages are released through the chunked path instead of leaking a slot)
     base = mmc.ScatterConfig(accessblocksize=2 * GiB, resetfreedpages=True, flags=mmc.FLAG_FIX_PAGESIZE_LEAK)
     if args.no_tune:
-        over = {k: v for k, v in (("xcd_slices", args.xcd_slices), ("regionsize", args.regionsize)) if v is not None}
+        over = {k: v for k, v in (("xcd_slices", args.xcd_slices), ("regionsize", args.regionsize),
+                                  ("lane_requests", args.lane_requests)) if v is not None}
         best = autotune.TunePoint(**{**STORED_POINT.__dict__, **over})
         tuned_points = 0
     else:
@@ -402,7 +415,7 @@ def main():
     mal_avg_ms = float(np.mean(mal_ms[1:]))
     # HBM bytes per malloc launch from the committed counter passes at this point
     run = counter_run(best)
-    traffic = run["waveMallocKernel"].get("hbm_bytes_per_launch") if run else None
+    traffic = run[kernel_names(best)[0]].get("hbm_bytes_per_launch") if run else None
     achieved_gbs = N_LANES * MALLOC_BYTES_PER_REQUEST / (mal_avg_ms / 1e3) / 1e9
     atomic_roofline = atomic_roofline_of(best, mal_avg_ms, ceiling)
 
@@ -470,7 +483,8 @@ def main():
             "workload": "mixed-size batch: 1,048,576 requests/step, sizes 16..4096 B (splitmix64 seeds 1,2,3), 4 GiB heap",
             "form": "batched+autotuned (wave-cooperative, hipGraph malloc+free per step)",
             "pagesize": best.pagesize, "regionsize": best.regionsize, "aggregation": best.aggregation,
-            "persistent": best.persistent, "xcd_slices": best.xcd_slices, "resetfreedpages": True,
+            "persistent": best.persistent, "xcd_slices": best.xcd_slices, "lane_requests": best.lane_requests,
+            "block_threads": 1024, "resetfreedpages": True,
             "fix_pagesize_leak": True,
             "autotune_points": tuned_points,
         },
@@ -483,7 +497,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "waveMallocKernel", "avg_launch_ms": mal_avg_ms,
+            "kernel": kernel_names(best)[0], "avg_launch_ms": mal_avg_ms,
             "bytes_per_request": MALLOC_BYTES_PER_REQUEST,
         },
         "atomic_roofline": atomic_roofline,

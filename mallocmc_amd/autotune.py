@@ -1,7 +1,8 @@
 """Autotuner of the batched (wave-cooperative) heap path.
 
 Sweeps heap geometry (pagesize, regionsize) and wave tuning (aggregation
-width, persistent launch, XCD slicing) on the caller's workload, measures
+width, persistent launch, XCD slicing, requests per lane) on the caller's
+workload, measures
 mallocs+frees/s with HIP events on the launch stream, checks every candidate
 against the verify_heap invariants, and returns the fastest valid point.
 Candidates that return more nulls than the reference point, or break an
@@ -25,6 +26,7 @@ class TunePoint:
     aggregation: int
     persistent: bool
     xcd_slices: int
+    lane_requests: int = 1
     ops_per_s: float = 0.0
     nulls: int = 0
     valid: bool = False
@@ -34,7 +36,8 @@ class TunePoint:
         return dataclasses.replace(base, pagesize=self.pagesize, regionsize=self.regionsize)
 
     def tuning(self) -> mmc.Tuning:
-        return mmc.Tuning(aggregation=self.aggregation, persistent=self.persistent, xcd_slices=self.xcd_slices)
+        return mmc.Tuning(aggregation=self.aggregation, persistent=self.persistent, xcd_slices=self.xcd_slices,
+                          lane_requests=self.lane_requests)
 
 
 DEFAULT_SPACE = dict(
@@ -43,6 +46,7 @@ DEFAULT_SPACE = dict(
     aggregation=(1, 16, 64),
     persistent=(False, True),
     xcd_slices=(0, 8),
+    lane_requests=(1, 2, 4),
 )
 
 
@@ -84,8 +88,8 @@ def measure(alloc: mmc.ScatterAllocator, size_batches, tuning: mmc.Tuning, round
 
 
 def _same_point(a: TunePoint, b: TunePoint) -> bool:
-    return (a.pagesize, a.regionsize, a.aggregation, bool(a.persistent), a.xcd_slices) == (
-        b.pagesize, b.regionsize, b.aggregation, bool(b.persistent), b.xcd_slices)
+    return (a.pagesize, a.regionsize, a.aggregation, bool(a.persistent), a.xcd_slices, a.lane_requests) == (
+        b.pagesize, b.regionsize, b.aggregation, bool(b.persistent), b.xcd_slices, b.lane_requests)
 
 
 def tune(base: mmc.ScatterConfig, heap_bytes: int, size_batches, space=None, log=None, max_nulls: int = 0,
@@ -110,8 +114,11 @@ def tune(base: mmc.ScatterConfig, heap_bytes: int, size_batches, space=None, log
             continue
         try:
             slots0 = alloc.getAvailableSlots(64)
-            for agg, pers, xcd in itertools.product(space["aggregation"], space["persistent"], space["xcd_slices"]):
-                pt = TunePoint(pagesize, regionsize, agg, pers, xcd)
+            for agg, pers, xcd, lr in itertools.product(space["aggregation"], space["persistent"],
+                                                        space["xcd_slices"], space.get("lane_requests", (1,))):
+                if pers and lr != 1:
+                    continue  # persistent launches serve one request per lane
+                pt = TunePoint(pagesize, regionsize, agg, pers, xcd, lr)
                 t0 = time.perf_counter()
                 pt.ops_per_s, pt.nulls, pt.valid = measure(alloc, size_batches, pt.tuning())
                 pt.seconds = time.perf_counter() - t0

@@ -52,3 +52,23 @@ def test_preferred_point_wins_a_tie(monkeypatch):
     # outside the margin the fastest point is kept
     best, _ = autotune.tune(base, 1 << 30, [], space=_space(), prefer=prefer, tie=0.01)
     assert (best.pagesize, best.regionsize, best.xcd_slices) == (65536, 32, 0)
+
+
+def test_requests_per_lane_are_swept_for_non_persistent_points_only(monkeypatch):
+    monkeypatch.setattr(mmc, "ScatterAllocator", _FakeAlloc)
+    seen = []
+
+    def measure(alloc, batches, tuning, rounds=2, timed=1):
+        seen.append((tuning.persistent, tuning.lane_requests))
+        return (2e9 if tuning.lane_requests == 2 else 1e9), 0, True
+
+    monkeypatch.setattr(autotune, "measure", measure)
+    base = mmc.ScatterConfig(resetfreedpages=True)
+    space = dict(pagesize=(65536,), regionsize=(16,), aggregation=(64,), persistent=(False, True),
+                 xcd_slices=(0,), lane_requests=(1, 2, 4))
+    best, points = autotune.tune(base, 1 << 30, [], space=space)
+    # persistent launches serve one request per lane: one point, not three
+    assert len(points) == 3 + 1
+    assert {(p.persistent, p.lane_requests) for p in points} == {(False, 1), (False, 2), (False, 4), (True, 1)}
+    assert best.lane_requests == 2 and not best.persistent
+    assert best.tuning().lane_requests == 2
