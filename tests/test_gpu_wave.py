@@ -168,6 +168,39 @@ def test_wave_batch_output_slots(mmc, cuda, no_partition, block_threads, lane_re
     a.close()
 
 
+@pytest.mark.parametrize("lane_requests", [1, 2, 4])
+def test_wave_overcommitted_batch(mmc, cuda, lane_requests):
+    """A batch asking for eight times the heap: most requests get null, the
+    served ones are intact and disjoint, and free-all restores the slots.
+    Covers the groups whose claims run out of pages (ranks left unserved)."""
+    torch = cuda
+    t = mmc.Tuning(aggregation=64, lane_requests=lane_requests)
+    cfg = mmc.ScatterConfig(pagesize=65536, accessblocksize=64 * MiB, regionsize=16, resetfreedpages=True)
+    a = mmc.ScatterAllocator(cfg, 64 * MiB)
+    s0 = [a.getAvailableSlots(s) for s in (16, 1000, 4096)]
+    n = 262144
+    sizes = torch.from_numpy(wl.mixed_sizes(5, n).astype(np.int32)).cuda()
+    print(f"overcommitted batch lane_requests={lane_requests}", flush=True)
+    offs = a.wave_malloc_batch(sizes, tuning=t)
+    a.write_pattern(offs, sizes, salt=4)
+    r = a.verify_pattern(offs, sizes, salt=4)
+    assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0, r
+    served = n - r["nulls"]
+    print(f"  served {served} of {n}", flush=True)
+    # the heap holds about 64 MiB of chunks: a small fraction of the ~540 MB asked for
+    assert 10000 < served < n // 4, r
+    assert no_overlap(torch, offs, sizes)
+    # the heap is full for the sizes asked: at most a few pages (under 1% of
+    # the ~1,000) left free, which concurrent claims passed over
+    free_pages = a.getAvailableSlots(65536)
+    print(f"  free pages left {free_pages}", flush=True)
+    assert free_pages <= 10
+    a.wave_free_batch(offs, tuning=t)
+    torch.cuda.synchronize()
+    assert [a.getAvailableSlots(s) for s in (16, 1000, 4096)] == s0
+    a.close()
+
+
 def test_wave_multipage_fill_to_null(mmc, cuda):
     """Two-page requests until the heap is full, through the wave path: every
     wave claims up to 64 runs, the page-based cursor follows the claimed runs
