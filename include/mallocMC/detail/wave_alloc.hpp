@@ -171,6 +171,9 @@ namespace mallocMC
                     return 0u;
                 if(static_cast<u32>(hw::popc(x)) <= n)
                     return x;
+                if((x & (x + 1u)) == 0u)
+                    return (1u << n) - 1u; // x = 0..01..1 (a fresh page's free bits): n < popc(x) <= 32
+
                 const u32 pos = selectBit(x, n - 1u); // position of the n-th set bit
                 return pos >= 31u ? x : (x & ((2u << pos) - 1u));
             }
@@ -281,14 +284,28 @@ namespace mallocMC
 
         //! Hands the chunks of `bits` (in bit order, chunk size cs, page data
         //! at `base`) to the group's ranks [from, from + n).
+        //! `bits` is the same in every lane; when its set bits are the lowest
+        //! ones (chunks 0 .. n-1 of a page claimed fresh), the r-th is bit r
+        //! and the bit search is skipped.
         template<u32 M>
         MALLOCMC_D void handOut(const GroupRanks<M>& g, u32 from, u32 n, char* base, u32 bits, u32 cs,
                                 char* (&out)[M])
         {
+            if((bits & (bits + 1u)) == 0u)
+            {
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    const u32 r = g.rank[j] - from; // wraps to a huge value below `from` and for kNoRank
+                    if(r < n)
+                        out[j] = base + static_cast<u64>(r) * cs;
+                }
+                return;
+            }
 #pragma unroll
             for(u32 j = 0; j < M; ++j)
             {
-                const u32 r = g.rank[j] - from; // wraps to a huge value below `from` and for kNoRank
+                const u32 r = g.rank[j] - from;
                 if(r < n)
                     out[j] = base + static_cast<u64>(wave::selectBit(bits, r)) * cs;
             }
