@@ -40,11 +40,19 @@ NO_TILE = 0xFFFFFFFF
 
 
 def make_workload(n_heaps: int = 64, seed: int = 11, lo: int = 1024, hi: int = 262144, s: float = 1.2,
-                  pad: int = 16):
+                  pad: int = 16, small_first: bool = True):
     """`pad`: each heap's tile list is padded with NO_TILE entries to a
     multiple of `pad` tiles, so that no workgroup of the workgroup form (4, 8
     or 16 tiles) straddles two heaps and every one partitions its requests;
-    1 pads nothing."""
+    1 pads nothing.  `small_first`: the tile list holds the heaps in order of
+    slot count, smallest first (else in heap order).  A launch dispatches its
+    workgroups in tile order; with the large heaps last, their workgroups are
+    dispatched as earlier ones finish, not all at once, and fewer of one
+    heap's mallocs run before the frees of the same step: on the 64-heap
+    workload 1.59e9 against 1.37e9 mallocs+frees/s, 0.43M against 0.53M
+    nulls per 100 steps (largest first: 1.62e9 but 1.04M nulls; heaps
+    interleaved one workgroup at a time: 1.42e9, 0.35M).  The slots and
+    their coins do not depend on the order."""
     rng = np.random.default_rng(seed)
     u = rng.random(n_heaps)
     slots = np.floor(lo * (hi / lo) ** u).astype(np.int64)
@@ -55,7 +63,8 @@ def make_workload(n_heaps: int = 64, seed: int = 11, lo: int = 1024, hi: int = 2
         t = np.arange(st, st + n, 64, dtype=np.uint64)
         extra = (-len(t)) % pad
         per_heap.append(np.concatenate([t, np.full(extra, NO_TILE, dtype=np.uint64)]))
-    tiles = np.concatenate(per_heap).astype(np.uint32)
+    order = np.argsort(slots, kind="stable") if small_first else np.arange(n_heaps)
+    tiles = np.concatenate([per_heap[h] for h in order]).astype(np.uint32)
     table = wl.ZIPF_SIZES
     w = 1.0 / np.arange(1, len(table) + 1) ** s
     cdf = np.minimum(np.round(np.cumsum(w / w.sum()) * 2 ** 32), 2 ** 32 - 1).astype(np.uint64)

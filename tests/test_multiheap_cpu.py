@@ -7,9 +7,10 @@ import pytest
 from mallocmc_amd import multiheap as mh
 
 
+@pytest.mark.parametrize("small_first", [True, False])
 @pytest.mark.parametrize("pad", [1, 4, 16])
-def test_workload_tiles_cover_each_heap_once(pad):
-    work = mh.make_workload(n_heaps=8, lo=100, hi=1000, pad=pad)
+def test_workload_tiles_cover_each_heap_once(pad, small_first):
+    work = mh.make_workload(n_heaps=8, lo=100, hi=1000, pad=pad, small_first=small_first)
     starts = np.concatenate([[0], np.cumsum(work.slots_per_heap)])
     # every slot of a heap lies in exactly one tile of that heap, tiles start 64 apart
     covered = np.zeros(len(work.slot_heap), dtype=np.int64)
@@ -28,6 +29,14 @@ def test_workload_tiles_cover_each_heap_once(pad):
         hs = {int(work.slot_heap[t]) for t in work.tiles[g:g + pad] if t != mh.NO_TILE}
         assert len(hs) <= 1, (g, hs)
     assert work.cdf[-1] == 2 ** 32 - 1 and (np.diff(work.cdf.astype(np.int64)) >= 0).all()
+    # heaps appear in one block each, smallest slot count first (else in heap order)
+    heaps = [int(work.slot_heap[t]) for t in work.tiles if t != mh.NO_TILE]
+    blocks = [h for i, h in enumerate(heaps) if i == 0 or heaps[i - 1] != h]
+    assert sorted(blocks) == list(range(8))
+    if small_first:
+        assert [work.slots_per_heap[h] for h in blocks] == sorted(work.slots_per_heap)
+    else:
+        assert blocks == list(range(8))
 
 
 class _FakeRun:
