@@ -1597,7 +1597,7 @@ namespace mallocMC
                 {
                     atom::and_(h.pteBitmask(page), ~pageBits);
                     old = atom::sub(h.pteCount(page), pageK);
-                    afterRelease(page, cs, old, pageK);
+                    releaseFinish(page, releaseIssue(page, cs, old, pageK));
                 }
             }
 
@@ -1618,20 +1618,43 @@ namespace mallocMC
             //! chunk size and count share one 64-bit word, so one CAS
             //! (cs, 0) -> (0, 0) resets the page if no one reserved it
             //! meanwhile.
-            MALLOCMC_D void afterRelease(u32 page, u32 cs, u32 old, u32 k) const
+            //! Two halves, so that a wave issues every group's loads and reset
+            //! before it waits for any of them: `releaseIssue` reads the
+            //! region's level and the access-block cursor (when the count
+            //! crossed the threshold) and issues the page reset;
+            //! `releaseFinish` writes the level and the cursor where they
+            //! change.  (One after the other, the two tests cost a wave two
+            //! dependent round trips more per free.)
+            struct Release
+            {
+                u32 level;
+                u32 cursor;
+                bool crossed;
+            };
+
+            MALLOCMC_D Release releaseIssue(u32 page, u32 cs, u32 old, u32 k) const
             {
                 const u32 thr = umax32(h.pdiv(h.pagesize / 2u, cs), 1u);
-                if(old >= thr && old - k < thr)
+                Release r{0u, 0u, old >= thr && old - k < thr};
+                if(r.crossed)
                 {
-                    u32* region = &h.regions[h.divR.div(page)];
-                    if(atom::load(region) != 0u)
-                        atom::exch(region, 0u);
-                    const u32 block = h.divPpb.div(page);
-                    if(block < atom::load(&h.scalars->firstfreeblock))
-                        atom::min(&h.scalars->firstfreeblock, block);
+                    r.level = atom::load(&h.regions[h.divR.div(page)]);
+                    r.cursor = atom::load(&h.scalars->firstfreeblock);
                 }
                 if(h.resetfreedpages && old == k)
                     atom::cas(h.ptePair(page), u64{cs}, u64{0});
+                return r;
+            }
+
+            MALLOCMC_D void releaseFinish(u32 page, const Release& r) const
+            {
+                if(!r.crossed)
+                    return;
+                if(r.level != 0u)
+                    atom::exch(&h.regions[h.divR.div(page)], 0u);
+                const u32 block = h.divPpb.div(page);
+                if(block < r.cursor)
+                    atom::min(&h.scalars->firstfreeblock, block);
             }
 
             //! Wave free of M pointers per lane (slot j of lane l is the wave's
@@ -1807,10 +1830,17 @@ namespace mallocMC
                         old[j] = atom::sub(h.pteCount(page[j]), pageK[j]);
                     }
                 }
+                Release rel[M];
 #pragma unroll
                 for(u32 j = 0; j < M; ++j)
+                {
+                    rel[j] = Release{0u, 0u, false};
                     if(pageLeader[j])
-                        afterRelease(page[j], cs[j], old[j], pageK[j]);
+                        rel[j] = releaseIssue(page[j], cs[j], old[j], pageK[j]);
+                }
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                    releaseFinish(page[j], rel[j]);
             }
         };
     } // namespace detail
