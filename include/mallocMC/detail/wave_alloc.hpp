@@ -1312,9 +1312,11 @@ namespace mallocMC
                 // the access-block cursor, read once for all groups of the wave
                 // (the reference reads it per request; any recent value is a
                 // valid start, Scatter.hpp:557-560); issued first so that its
-                // round trip overlaps the size loads
+                // round trip overlaps the size loads.  A heap of one access
+                // block keeps it at 0 (no walk enters another block), and its
+                // first scan window need not wait for it.
                 u32 ffb = 0u;
-                if(me == 0u)
+                if(me == 0u && h.accessblocks > 1u)
                     ffb = atom::load(&h.scalars->firstfreeblock);
                 Dist dist;
                 const u32 b = dist.collect(h, h.pad(bytesIn));
@@ -1394,8 +1396,9 @@ namespace mallocMC
             MALLOCMC_D void mallocMany(const u32 (&bytesIn)[M], void* (&res)[M]) const
             {
                 const u32 me = hw::laneid();
+                // the access-block cursor, as in malloc()
                 u32 ffb = 0u;
-                if(me == 0u)
+                if(me == 0u && h.accessblocks > 1u)
                     ffb = atom::load(&h.scalars->firstfreeblock);
                 u32 b[M];
                 char* result[M];
@@ -1639,7 +1642,9 @@ namespace mallocMC
                 if(r.crossed)
                 {
                     r.level = atom::load(&h.regions[h.divR.div(page)]);
-                    r.cursor = atom::load(&h.scalars->firstfreeblock);
+                    // one access block: the cursor is 0 and stays there
+                    if(h.accessblocks > 1u)
+                        r.cursor = atom::load(&h.scalars->firstfreeblock);
                 }
                 if(h.resetfreedpages && old == k)
                     atom::cas(h.ptePair(page), u64{cs}, u64{0});
