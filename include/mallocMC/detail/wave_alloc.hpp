@@ -883,18 +883,22 @@ namespace mallocMC
                         if(f < checklevel && f + 1u <= R)
                             atom::cas(level, f, f + 1u);
                     }
-                    // access-block cursor over the regions passed
-                    u32 abid = 0u;
-                    if(me < passed)
+                    // access-block cursor over the regions passed (none to move
+                    // on a heap of one access block)
+                    if(h.accessblocks > 1u)
                     {
-                        const u32 rg = wrap(region0 + from + skipped + me + 1u);
-                        const u32 q = h.divPpb.div(rg * R);
-                        if(q * h.pagesperblock == rg * R)
-                            abid = q;
+                        u32 abid = 0u;
+                        if(me < passed)
+                        {
+                            const u32 rg = wrap(region0 + from + skipped + me + 1u);
+                            const u32 q = h.divPpb.div(rg * R);
+                            if(q * h.pagesperblock == rg * R)
+                                abid = q;
+                        }
+                        abid = wave::max(abid);
+                        if(abid != 0u && me == 0u && abid > atom::load(&h.scalars->firstfreeblock))
+                            atom::store(&h.scalars->firstfreeblock, abid);
                     }
-                    abid = wave::max(abid);
-                    if(abid != 0u && me == 0u && abid > atom::load(&h.scalars->firstfreeblock))
-                        atom::store(&h.scalars->firstfreeblock, abid);
                     skipped += passed;
                     if(first != 0xffffffffu)
                         break;
@@ -1112,18 +1116,21 @@ namespace mallocMC
                         }
                         // access-block cursor: advanced when the walk enters a new block
                         // (Scatter.hpp:620-629); lane i looks at the region after region i
-                        // of this round
-                        u32 abid = 0u;
-                        if(me < nreg)
+                        // of this round (a heap of one access block has no other block)
+                        if(h.accessblocks > 1u)
                         {
-                            const u32 rg = wrap(region0 + done + me + 1u);
-                            const u32 q = h.divPpb.div(rg * R);
-                            if(q * ppb == rg * R)
-                                abid = q;
+                            u32 abid = 0u;
+                            if(me < nreg)
+                            {
+                                const u32 rg = wrap(region0 + done + me + 1u);
+                                const u32 q = h.divPpb.div(rg * R);
+                                if(q * ppb == rg * R)
+                                    abid = q;
+                            }
+                            abid = wave::max(abid);
+                            if(abid != 0u && me == 0u && abid > atom::load(&h.scalars->firstfreeblock))
+                                atom::store(&h.scalars->firstfreeblock, abid);
                         }
-                        abid = wave::max(abid);
-                        if(abid != 0u && me == 0u && abid > atom::load(&h.scalars->firstfreeblock))
-                            atom::store(&h.scalars->firstfreeblock, abid);
                         // a window with no candidate page at all: skip ahead over
                         // the regions that have none either, 256 pages per round trip
                         if(!windowHadCand && need.open() && R < 64u && t.fastForward != 0u && done + W < numregions)
@@ -1346,7 +1353,6 @@ namespace mallocMC
                     want = false;
                 u64 pending = hw::ballot(want);
                 const u32 cls = want ? sizeClass(h, b, t.classSplit) : 0xffffffffu;
-                const u32 minAlloc = umax32(b, h.paddedMinChunk);
                 ffb = wave::lane(ffb, 0u);
                 while(pending != 0u)
                 {
@@ -1366,8 +1372,8 @@ namespace mallocMC
                                | (static_cast<u64>(wave::lowestBits(hi, t.aggregation - nlo)) << 32));
                     }
                     const bool member = ((group >> me) & 1u) != 0u;
-                    const u32 c = wave::max(member ? minAlloc : 0u);
                     const u32 bmax = wave::max(member ? b : 0u);
+                    const u32 c = umax32(bmax, h.paddedMinChunk); // the largest member's chunk
                     const u32 bmin = wave::min(member ? b : 0xffffffffu);
                     const u32 maxcs = umin32(h.pagesize, umax32(h.wastefactor * bmin, h.paddedMinChunk));
                     char* mine[1] = {nullptr};
@@ -1490,17 +1496,16 @@ namespace mallocMC
                     }
                     need.K = K;
                     need.got = 0u;
-                    u32 cMax = 0u, bMax = 0u, bMin = 0xffffffffu;
+                    u32 bMax = 0u, bMin = 0xffffffffu;
 #pragma unroll
                     for(u32 j = 0; j < M; ++j)
                         if(need.rank.rank[j] != kNoRank)
                         {
-                            cMax = umax32(cMax, umax32(b[j], h.paddedMinChunk));
                             bMax = umax32(bMax, b[j]);
                             bMin = umin32(bMin, b[j]);
                         }
-                    const u32 c = wave::max(cMax);
                     const u32 bmax = wave::max(bMax);
+                    const u32 c = umax32(bmax, h.paddedMinChunk); // the largest member's chunk
                     const u32 bmin = wave::min(bMin);
                     const u32 maxcs = umin32(h.pagesize, umax32(h.wastefactor * bmin, h.paddedMinChunk));
                     // chunks go straight to `result`: only served ranks are written
