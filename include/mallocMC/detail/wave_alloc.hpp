@@ -349,6 +349,17 @@ namespace mallocMC
                 return L;
             }
 
+            //! pageCapacity(cs); the group's own chunk size c, whose capacity
+            //! capC the walk computed once, skips the divisions (a divergent
+            //! branch that no lane takes costs nothing).
+            MALLOCMC_D u32 capacityOf(u32 cs, u32 c, u32 capC) const
+            {
+                u32 cap = capC;
+                if(cs != c)
+                    cap = h.pageCapacity(cs);
+                return cap;
+            }
+
             //! True for a chunk size the page reset applies to: chunked pages,
             //! and pagesize chunks with the pagesize-leak fix.  Pages of a
             //! multi-page allocation (chunk size above the page, count 0 while
@@ -581,7 +592,7 @@ namespace mallocMC
             //! view of them was stale).
             template<u32 M>
             MALLOCMC_D u32 claimFlat(u32 pageL, u32 pcs, u32 pcount, u32 pbits, u64 used, u64 fresh, u32 rot,
-                                     u32 c, u32 k, const GroupRanks<M>& g, u64& chosen, u64& failed,
+                                     u32 c, u32 capC, u32 k, const GroupRanks<M>& g, u64& chosen, u64& failed,
                                      char* (&out)[M]) const
             {
                 const u32 me = hw::laneid();
@@ -589,7 +600,7 @@ namespace mallocMC
                 const bool isUsed = ((used >> me) & 1u) != 0u;
                 const bool isFresh = ((fresh >> me) & 1u) != 0u;
                 const u32 csl = isUsed ? pcs : c;
-                const u32 capl = (isUsed || isFresh) ? umin32(h.pdiv(P, csl), 32u) : 0u;
+                const u32 capl = (isUsed || isFresh) ? capacityOf(csl, c, capC) : 0u;
                 const u32 selBits = capl >= 32u ? ~0u : ((1u << capl) - 1u);
                 const u32 freeBits = (isUsed || isFresh) ? (~pbits & selBits) : 0u;
                 const u32 room = isUsed ? (capl > pcount ? capl - pcount : 0u) : capl;
@@ -697,13 +708,13 @@ namespace mallocMC
             //! many ranks it served, lowest first; `failed` gets the chosen
             //! pages that yielded no chunk.
             template<u32 M>
-            MALLOCMC_D u32 claimFresh(u32 pageL, u64 fresh, u32 rot, u32 c, u32 k, const GroupRanks<M>& g,
-                                      u64& chosen, u64& failed, char* (&out)[M]) const
+            MALLOCMC_D u32 claimFresh(u32 pageL, u64 fresh, u32 rot, u32 c, u32 capC, const Layout& LC, u32 k,
+                                      const GroupRanks<M>& g, u64& chosen, u64& failed, char* (&out)[M]) const
             {
                 constexpr u32 NW = M >= 2u ? 4u : 2u; // segment words one page's claim may touch
                 const u32 P = h.pagesize;
                 const bool isFresh = ((fresh >> hw::laneid()) & 1u) != 0u;
-                const u32 cap = h.pageCapacity(c);
+                const u32 cap = capC;
                 // order among the candidates: from the rotation point, then the rest
                 const u64 hiMask = rot < 64u ? (~0ull << rot) : 0ull;
                 const u64 below = hw::lanemask_lt();
@@ -714,7 +725,8 @@ namespace mallocMC
                 const u32 perPage = umin32(cap, 32u * NW);
                 const u32 pos = order * perPage;
                 const u32 take = (isFresh && pos < k) ? umin32(perPage, k - pos) : 0u;
-                const Layout L = layout(pageL, c);
+                Layout L = LC; // the chunk size's layout; this lane's page's words
+                L.words = h.onPageMasks(pageL, LC.nWords);
                 // the lowest `take` chunks: word w holds 32 (w < full) or
                 // `additional` (w == full) of them
                 u32 m[NW];
@@ -818,7 +830,7 @@ namespace mallocMC
             //! regionsize < 64 only.
             template<class Wrap>
             MALLOCMC_D u32 fastForward(u32 region0, u32 from, u32 numregions, u32 checklevel, u32 bmax, u32 maxcs,
-                                       Wrap wrap) const
+                                       u32 c, u32 capC, Wrap wrap) const
             {
                 const u32 me = hw::laneid();
                 const u32 R = h.regionsize;
@@ -860,7 +872,7 @@ namespace mallocMC
                             continue;
                         const u32 pc = css[l];
                         const bool cand = pc == 0u
-                            || (pc >= bmax && pc <= maxcs && pc <= h.pagesize && cnts[l] < h.pageCapacity(pc));
+                            || (pc >= bmax && pc <= maxcs && pc <= h.pagesize && cnts[l] < capacityOf(pc, c, capC));
                         if(cand)
                             first = umin32(first, regRel[l]);
                     }
@@ -963,6 +975,9 @@ namespace mallocMC
                 // claims on several pages per round trip (claimFlat, claimFresh);
                 // with flat_single every page is claimed on its own (claimOnPage)
                 const bool multi = t.flatFast != 0u;
+                // the group's chunk size: capacity and segment layout, once
+                const u32 capC = h.pageCapacity(c);
+                const Layout LC = layout(0u, c);
                 for(u32 finder = 0; finder < 2u && need.open(); ++finder)
                 {
                     for(u32 done = 0; done < numregions && need.open(); done += W)
@@ -1002,7 +1017,7 @@ namespace mallocMC
                                     if(pcs == 0u)
                                         cand = true;
                                     else if(pcs >= bmax && pcs <= maxcs && pcs <= h.pagesize)
-                                        cand = pcount < h.pageCapacity(pcs);
+                                        cand = pcount < capacityOf(pcs, c, capC);
                                 }
                                 const u64 usedAll = hw::ballot(cand && pcs != 0u);
                                 const u64 freeAll = hw::ballot(cand && pcs == 0u);
@@ -1034,7 +1049,7 @@ namespace mallocMC
                                             while(need.open() && (used | cands) != 0u)
                                             {
                                                 u64 chosen = 0u, failed = 0u;
-                                                need.served(wave::uniform(claimFlat<M>(pageL, pcs, pcount, pbits, used, cands, rot, c,
+                                                need.served(wave::uniform(claimFlat<M>(pageL, pcs, pcount, pbits, used, cands, rot, c, capC,
                                                                          need.remaining(), need.ranks(), chosen,
                                                                          failed, out)));
                                                 used &= ~chosen;
@@ -1070,7 +1085,7 @@ namespace mallocMC
                                             while(t.freshFast != 0u && need.open() && cands != 0u && !staleK)
                                             {
                                                 u64 chosen = 0u, failed = 0u;
-                                                need.served(wave::uniform(claimFresh<M>(pageL, cands, rot, c, need.remaining(),
+                                                need.served(wave::uniform(claimFresh<M>(pageL, cands, rot, c, capC, LC, need.remaining(),
                                                                           need.ranks(), chosen, failed, out)));
                                                 cands &= ~chosen;
                                                 staleK = staleK || failed != 0u;
@@ -1134,7 +1149,7 @@ namespace mallocMC
                         // a window with no candidate page at all: skip ahead over
                         // the regions that have none either, 256 pages per round trip
                         if(!windowHadCand && need.open() && R < 64u && t.fastForward != 0u && done + W < numregions)
-                            done += fastForward(region0, done + W, numregions, checklevel, bmax, maxcs, wrap);
+                            done += fastForward(region0, done + W, numregions, checklevel, bmax, maxcs, c, capC, wrap);
                     }
                     checklevel = R + 1u;
                 }
