@@ -130,14 +130,35 @@ namespace mallocMC
             {
                 return reduce(v, [](u32 a, u32 b) { return a | b; });
             }
-            //! Exclusive prefix sum over lanes of values below 64, bit by bit
-            //! with a ballot and an mbcnt per bit.
-            MALLOCMC_D u32 prefix6(u32 v)
+            //! DPP move whose lanes outside RowMask, and lanes whose source
+            //! lies outside their row, read 0 (for sums).
+            template<int Ctrl, int RowMask = 0xf>
+            MALLOCMC_D u32 dppZero(u32 v)
             {
-                u32 p = 0u;
-                for(u32 b = 0; b < 6u; ++b)
-                    p += hw::rankBelow(hw::ballot(((v >> b) & 1u) != 0u)) << b;
-                return p;
+                return static_cast<u32>(
+                    __builtin_amdgcn_update_dpp(0, static_cast<int>(v), Ctrl, RowMask, 0xf, true));
+            }
+            //! DPP controls row_shr:1, 2, 4, 8 (shift within a row of 16 lanes).
+            constexpr int kRowShr1 = 0x111;
+            constexpr int kRowShr2 = 0x112;
+            constexpr int kRowShr4 = 0x114;
+            constexpr int kRowShr8 = 0x118;
+            //! Exclusive prefix sum over the 64 lanes (modulo 2^32): a
+            //! Hillis-Steele scan inside each row of 16 lanes (shifts 1, 2, 4,
+            //! 8), then rows 1 and 3 add the last lane of rows 0 and 2, and
+            //! rows 2 and 3 add lane 31.  Six DPP additions; the bit-by-bit
+            //! form (a ballot and an mbcnt per bit of the value) took about
+            //! thirty instructions for values below 64.
+            MALLOCMC_D u32 exclusiveSum(u32 v)
+            {
+                u32 s = v;
+                s += dppZero<kRowShr1>(s);
+                s += dppZero<kRowShr2>(s);
+                s += dppZero<kRowShr4>(s);
+                s += dppZero<kRowShr8>(s);
+                s += dppZero<kRowBcast15, 0xa>(s);
+                s += dppZero<kRowBcast31, 0xc>(s);
+                return s - v;
             }
             //! Value of lane `l` (l uniform).
             MALLOCMC_D u32 lane(u32 v, u32 l)
@@ -540,7 +561,7 @@ namespace mallocMC
                     const u32 freeBits = owner ? (~snap & sel) : 0u;
                     const u32 freeCnt = hw::popc(freeBits);
                     // exclusive prefix of free counts (<= 32) over words 0..me-1
-                    const u32 prefix = wave::prefix6(freeCnt);
+                    const u32 prefix = wave::exclusiveSum(freeCnt);
                     u32 take = 0u;
                     if(owner && prefix < remaining)
                         take = umin32(freeCnt, remaining - prefix);
@@ -611,10 +632,10 @@ namespace mallocMC
                 const bool inUlo = isUsed && !inUhi;
                 const bool inFhi = isFresh && ((hiMask >> me) & 1u);
                 const bool inFlo = isFresh && !inFhi;
-                const u32 pUhi = wave::prefix6(inUhi ? f : 0u);
-                const u32 pUlo = wave::prefix6(inUlo ? f : 0u);
-                const u32 pFhi = wave::prefix6(inFhi ? f : 0u);
-                const u32 pFlo = wave::prefix6(inFlo ? f : 0u);
+                const u32 pUhi = wave::exclusiveSum(inUhi ? f : 0u);
+                const u32 pUlo = wave::exclusiveSum(inUlo ? f : 0u);
+                const u32 pFhi = wave::exclusiveSum(inFhi ? f : 0u);
+                const u32 pFlo = wave::exclusiveSum(inFlo ? f : 0u);
                 const u32 tUhi = wave::lane(pUhi + (inUhi ? f : 0u), 63u);
                 const u32 tUlo = wave::lane(pUlo + (inUlo ? f : 0u), 63u);
                 const u32 tFhi = wave::lane(pFhi + (inFhi ? f : 0u), 63u);

@@ -48,7 +48,7 @@ int mmc_bench_atomics(uint32_t* d_buf, uint64_t words, uint32_t pattern, uint32_
 int mmc_bench_copy(const void* d_src, void* d_dst, uint64_t bytes, void* stream);
 
 /* Self-test of the wave primitives the wave path is built on (DPP max, min
- * and or reductions, the ballot prefix sum), no reference counterpart:
+ * and or reductions, the DPP prefix sum), no reference counterpart:
  * `waves` wavefronts draw seeded values (from `seed`), compare each primitive
  * with a lane-by-lane loop and add their mismatches to d_result[0..3]
  * (uint32: max, min, or, prefix; zeroed by the call). */

@@ -164,7 +164,7 @@ namespace
         const u32 shift = blockIdx.x % 33u;
         const u32 v = static_cast<u32>(z) >> shift;
         const u32 small = static_cast<u32>(z >> 40) % 33u; // 0..32, the prefix domain
-        u32 bmax = 0u, bmin = 0xffffffffu, bor = 0u, bpre = 0u;
+        u32 bmax = 0u, bmin = 0xffffffffu, bor = 0u, bpre = 0u, vpre = 0u;
         for(u32 l = 0; l < 64u; ++l)
         {
             const u32 o = static_cast<u32>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(l)));
@@ -173,11 +173,13 @@ namespace
             bmin = o < bmin ? o : bmin;
             bor |= o;
             bpre += l < me ? so : 0u;
+            vpre += l < me ? o : 0u; // modulo 2^32
         }
         const u32 m0 = hw::ballot(wave::max(v) != bmax) != 0u ? 1u : 0u;
         const u32 m1 = hw::ballot(wave::min(v) != bmin) != 0u ? 1u : 0u;
         const u32 m2 = hw::ballot(wave::bor(v) != bor) != 0u ? 1u : 0u;
-        const u32 m3 = static_cast<u32>(hw::popc(hw::ballot(wave::prefix6(small) != bpre)));
+        const u32 m3 = static_cast<u32>(hw::popc(hw::ballot(wave::exclusiveSum(small) != bpre)))
+            + static_cast<u32>(hw::popc(hw::ballot(wave::exclusiveSum(v) != vpre)));
         if(me == 0u)
         {
             if(m0)

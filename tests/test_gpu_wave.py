@@ -473,8 +473,9 @@ def test_multi_heap_steps_with_multipage_requests(mmc, cuda):
 
 
 def test_wave_primitives_selftest(mmc, cuda):
-    """The DPP reductions (max, min, or) and the ballot prefix sum agree with
-    lane-by-lane loops on 4096 seeded waves, value ranges 32..0 bits wide."""
+    """The DPP reductions (max, min, or) and the DPP prefix sum agree with
+    lane-by-lane loops on 4096 seeded waves, value ranges 32..0 bits wide
+    (the prefix sum also on values 0..32, the range the claims use)."""
     import ctypes
     torch = cuda
     res = torch.zeros(4, dtype=torch.int32, device="cuda")
