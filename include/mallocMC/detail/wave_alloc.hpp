@@ -130,6 +130,14 @@ namespace mallocMC
             {
                 return reduce(v, [](u32 a, u32 b) { return a | b; });
             }
+            //! Sum over the 64 lanes (modulo 2^32).  reduce() suits a sum as
+            //! well: the lane it returns (63) only ever reads lanes that
+            //! already hold exact partial sums, whatever the lanes outside a
+            //! row mask compute.
+            MALLOCMC_D u32 sum(u32 v)
+            {
+                return reduce(v, [](u32 a, u32 b) { return a + b; });
+            }
             //! DPP move whose lanes outside RowMask, and lanes whose source
             //! lies outside their row, read 0 (for sums).
             template<int Ctrl, int RowMask = 0xf>
@@ -159,6 +167,11 @@ namespace mallocMC
                 s += dppZero<kRowBcast15, 0xa>(s);
                 s += dppZero<kRowBcast31, 0xc>(s);
                 return s - v;
+            }
+            //! Inclusive prefix sum over the 64 lanes (modulo 2^32).
+            MALLOCMC_D u32 inclusiveSum(u32 v)
+            {
+                return exclusiveSum(v) + v;
             }
             //! Value of lane `l` (l uniform).
             MALLOCMC_D u32 lane(u32 v, u32 l)

@@ -107,17 +107,9 @@ namespace
         for(u32 j = 0; j < M; ++j)
             rank[j] = atomicAdd(&count[key[j]], 1u);
         __syncthreads();
-        // lane l: buckets 4l .. 4l+3, then an inclusive scan of the lane sums
+        // lane l: buckets 4l .. 4l+3, then a scan of the lane sums
         const u32 c0 = count[4u * me], c1 = count[4u * me + 1u], c2 = count[4u * me + 2u], c3 = count[4u * me + 3u];
-        const u32 sum = c0 + c1 + c2 + c3;
-        u32 incl = sum;
-        for(u32 d = 1u; d < 64u; d <<= 1)
-        {
-            const u32 up = __shfl_up(incl, d, 64);
-            if(me >= d)
-                incl += up;
-        }
-        const u32 excl = incl - sum;
+        const u32 excl = wave::exclusiveSum(c0 + c1 + c2 + c3);
         start[4u * me] = excl;
         start[4u * me + 1u] = excl + c0;
         start[4u * me + 2u] = excl + c0 + c1;
@@ -398,12 +390,9 @@ namespace
             }
         }
         // wave totals into the counters
-        for(int x = 32; x > 0; x >>= 1)
-        {
-            nMalloc += __shfl_xor(nMalloc, x, 64);
-            nFree += __shfl_xor(nFree, x, 64);
-            nNull += __shfl_xor(nNull, x, 64);
-        }
+        nMalloc = wave::sum(static_cast<u32>(nMalloc));
+        nFree = wave::sum(static_cast<u32>(nFree));
+        nNull = wave::sum(static_cast<u32>(nNull));
         if(me == 0u)
         {
             if(nMalloc)
@@ -530,12 +519,9 @@ namespace
             d.d_offsets[target] = h.offsetOf(p);
             d.d_sizes[target] = p == nullptr ? 0u : bytes;
         }
-        for(int x = 32; x > 0; x >>= 1)
-        {
-            nMalloc += __shfl_xor(nMalloc, x, 64);
-            nFree += __shfl_xor(nFree, x, 64);
-            nNull += __shfl_xor(nNull, x, 64);
-        }
+        nMalloc = wave::sum(static_cast<u32>(nMalloc));
+        nFree = wave::sum(static_cast<u32>(nFree));
+        nNull = wave::sum(static_cast<u32>(nNull));
         if(me == 0u)
         {
             if(nMalloc)

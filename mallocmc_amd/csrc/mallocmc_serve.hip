@@ -117,13 +117,7 @@ namespace
             const u32 len = arrived ? v.length[idx] : 0u;
             const u32 need = arrived ? divup(len, v.tokensPerBlock) * v.blockBytes : 0u;
             // inclusive prefix of the projected bytes over lanes 0..lane
-            u32 incl = need;
-            for(int d = 1; d < 64; d <<= 1)
-            {
-                const u32 o = __shfl_up(incl, static_cast<unsigned>(d), 64);
-                if(lane >= static_cast<u32>(d))
-                    incl += o;
-            }
+            const u32 incl = wave::inclusiveSum(need);
             const bool fits = arrived && reserved + incl <= v.budget && lane < top;
             // the leading run of lanes that fit is admitted, in order
             const u64 ok = hw::ballot(fits);
@@ -139,9 +133,7 @@ namespace
                     v.admitStep[idx] = static_cast<u32>(step);
             }
             const u32 waitLane = lane < count ? static_cast<u32>(step - v.arrival[idx]) : 0u;
-            u32 w = waitLane;
-            for(int d = 32; d > 0; d >>= 1)
-                w += __shfl_xor(w, d, 64);
+            const u32 w = wave::sum(waitLane);
             const u32 bytes = count ? wave::lane(incl, count - 1u) : 0u;
             reserved += bytes;
             top -= count;
@@ -278,12 +270,8 @@ namespace
                 atom::store(&v.freeStack[freeAt], slot);
             }
         }
-        u32 mallocs = needBlock ? 1u : 0u;
-        for(int d = 32; d > 0; d >>= 1)
-        {
-            mallocs += __shfl_xor(mallocs, d, 64);
-            nulls += __shfl_xor(nulls, d, 64);
-        }
+        const u32 mallocs = wave::sum(needBlock ? 1u : 0u);
+        nulls = wave::sum(nulls);
         if(hw::laneid() == 0u)
         {
             if(mallocs)
@@ -317,8 +305,7 @@ namespace
             wa.free(off == kNullOffset ? nullptr : v.heap.pages + off);
             freed += off != kNullOffset ? 1u : 0u;
         }
-        for(int d = 32; d > 0; d >>= 1)
-            freed += __shfl_xor(freed, d, 64);
+        freed = wave::sum(freed);
         if(lane == 0u)
             atom::add(&v.sc->counters[cFrees], static_cast<u64>(freed));
     }
@@ -377,13 +364,7 @@ namespace
                 const u32 el = e < nDone ? v.doneLen[list + e] : 0u;
                 const u32 er = e < nDone ? v.doneReq[list + e] : 0u;
                 const u32 n = divup(el, kTokensPerChunk);
-                u32 incl = n;
-                for(u32 d = 1u; d < 64u; d <<= 1)
-                {
-                    const u32 up = __shfl_up(incl, d, 64);
-                    if(lane >= d)
-                        incl += up;
-                }
+                const u32 incl = wave::inclusiveSum(n);
                 const u64 past = hw::ballot(c < base + incl); // entries whose range ends after c
                 if(past != 0u)
                 {
