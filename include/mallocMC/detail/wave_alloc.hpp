@@ -91,30 +91,31 @@ namespace mallocMC
             constexpr int kRowBcast15 = 0x142;
             constexpr int kRowBcast31 = 0x143;
 
+            //! DPP move whose lanes outside RowMask, and lanes whose source
+            //! lies outside their row, read 0.
             template<int Ctrl, int RowMask = 0xf>
-            MALLOCMC_D u32 dpp(u32 v)
+            MALLOCMC_D u32 dppZero(u32 v)
             {
-                // lanes outside RowMask keep v itself
-                return static_cast<u32>(__builtin_amdgcn_update_dpp(
-                    static_cast<int>(v),
-                    static_cast<int>(v),
-                    Ctrl,
-                    RowMask,
-                    0xf,
-                    false));
+                return static_cast<u32>(
+                    __builtin_amdgcn_update_dpp(0, static_cast<int>(v), Ctrl, RowMask, 0xf, true));
             }
 
-            //! Combines all 64 lanes with an associative, commutative op;
-            //! lane 63 ends with the result, which is returned to all lanes.
+            //! Combines all 64 lanes with an associative, commutative op whose
+            //! identity is 0 (unsigned max, or, sum); lane 63 ends with the
+            //! result, which is returned to all lanes.  Lanes a DPP step does
+            //! not feed combine with 0 and keep their value, and lane 63 only
+            //! ever reads lanes that hold exact partial results.  With 0 as
+            //! the moves' fill, the compiler folds each move into its
+            //! combining instruction.
             template<class Op>
             MALLOCMC_D u32 reduce(u32 v, Op op)
             {
-                v = op(v, dpp<kQuadXor1>(v));
-                v = op(v, dpp<kQuadXor2>(v)); // quad value in every lane
-                v = op(v, dpp<kRowRor4>(v));
-                v = op(v, dpp<kRowRor8>(v)); // row (16 lanes) value in every lane
-                v = op(v, dpp<kRowBcast15, 0xa>(v)); // rows 1, 3 add rows 0, 2
-                v = op(v, dpp<kRowBcast31, 0xc>(v)); // rows 2, 3 add lane 31
+                v = op(v, dppZero<kQuadXor1>(v));
+                v = op(v, dppZero<kQuadXor2>(v)); // quad value in every lane
+                v = op(v, dppZero<kRowRor4>(v));
+                v = op(v, dppZero<kRowRor8>(v)); // row (16 lanes) value in every lane
+                v = op(v, dppZero<kRowBcast15, 0xa>(v)); // rows 1, 3 add rows 0, 2
+                v = op(v, dppZero<kRowBcast31, 0xc>(v)); // rows 2, 3 add lane 31
                 return static_cast<u32>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
             }
 
@@ -130,21 +131,10 @@ namespace mallocMC
             {
                 return reduce(v, [](u32 a, u32 b) { return a | b; });
             }
-            //! Sum over the 64 lanes (modulo 2^32).  reduce() suits a sum as
-            //! well: the lane it returns (63) only ever reads lanes that
-            //! already hold exact partial sums, whatever the lanes outside a
-            //! row mask compute.
+            //! Sum over the 64 lanes (modulo 2^32).
             MALLOCMC_D u32 sum(u32 v)
             {
                 return reduce(v, [](u32 a, u32 b) { return a + b; });
-            }
-            //! DPP move whose lanes outside RowMask, and lanes whose source
-            //! lies outside their row, read 0 (for sums).
-            template<int Ctrl, int RowMask = 0xf>
-            MALLOCMC_D u32 dppZero(u32 v)
-            {
-                return static_cast<u32>(
-                    __builtin_amdgcn_update_dpp(0, static_cast<int>(v), Ctrl, RowMask, 0xf, true));
             }
             //! DPP controls row_shr:1, 2, 4, 8 (shift within a row of 16 lanes).
             constexpr int kRowShr1 = 0x111;
