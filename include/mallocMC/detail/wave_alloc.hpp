@@ -1748,11 +1748,15 @@ namespace mallocMC
                     }
                 }
 
-                // ---- grouping (registers only); the leader of a group is its
-                // lowest slot in (slot, lane) order
+                // ---- grouping (registers only), slot by slot: a group is the
+                // lanes of one slot j that free chunks of one page, led by its
+                // lowest lane.  (Groups across the slots, one per page, took
+                // more instructions per page than the extra group costs where
+                // a page's frees span two slots; the callers hand a wave its
+                // frees sorted by page index mod 256, so a page's frees lie in
+                // one slot or two neighbouring ones.)
                 bool pageLeader[M], segLeader[M];
                 u32 pageK[M], pageBits[M], segBits[M];
-                u64 pending[M];
 #pragma unroll
                 for(u32 j = 0; j < M; ++j)
                 {
@@ -1761,100 +1765,42 @@ namespace mallocMC
                     pageK[j] = 0u;
                     pageBits[j] = 0u;
                     segBits[j] = 0u;
-                    pending[j] = hw::ballot(simple[j]);
-                }
-                for(;;)
-                {
-                    u32 lj = M;
-#pragma unroll
-                    for(u32 j = M; j-- > 0u;)
-                        if(pending[j] != 0u)
-                            lj = j;
-                    if(lj == M)
-                        break;
-                    u64 lpend = 0u;
-                    u32 lp = 0u, lcs = 0u;
-#pragma unroll
-                    for(u32 j = 0; j < M; ++j)
-                        if(j == lj)
-                        {
-                            lpend = pending[j];
-                            lp = page[j];
-                            lcs = cs[j];
-                        }
-                    const u32 leader = hw::ffs(lpend) - 1u;
-                    const u32 lpage = wave::lane(lp, leader);
-                    // every member shares the page, hence its chunk size
-                    const bool hier = wave::lane(lcs, leader) <= h.hierarchyThreshold;
-                    u64 group[M];
-                    u32 k = 0u, bitsIn = 0u;
-#pragma unroll
-                    for(u32 j = 0; j < M; ++j)
+                    u64 pending = hw::ballot(simple[j]);
+                    while(pending != 0u)
                     {
-                        const bool member = ((pending[j] >> me) & 1u) != 0u && page[j] == lpage;
-                        group[j] = hw::ballot(member);
-                        k += static_cast<u32>(hw::popc(group[j]));
+                        const u32 leader = hw::ffs(pending) - 1u;
+                        const u32 lpage = wave::lane(page[j], leader);
+                        // every member shares the page, hence its chunk size
+                        const bool hier = wave::lane(cs[j], leader) <= h.hierarchyThreshold;
+                        const bool member = ((pending >> me) & 1u) != 0u && page[j] == lpage;
+                        const u64 group = hw::ballot(member);
                         // PTE bitmask bits to clear: chunk bits (flat) or segment hints (hierarchical)
-                        if(member)
-                            bitsIn |= hier ? (1u << (idx[j] / 32u)) : (1u << idx[j]);
-                    }
-                    const u32 bits = wave::bor(bitsIn);
-#pragma unroll
-                    for(u32 j = 0; j < M; ++j)
-                        if(j == lj && me == leader)
+                        const u32 bits
+                            = wave::bor(member ? (hier ? (1u << (idx[j] / 32u)) : (1u << idx[j])) : 0u);
+                        if(me == leader)
                         {
                             pageLeader[j] = true;
-                            pageK[j] = k;
+                            pageK[j] = static_cast<u32>(hw::popc(group));
                             pageBits[j] = bits;
                         }
-                    if(hier)
-                    {
-                        u64 segPending[M];
-#pragma unroll
-                        for(u32 j = 0; j < M; ++j)
-                            segPending[j] = group[j];
-                        for(;;)
+                        if(hier)
                         {
-                            u32 sj = M;
-#pragma unroll
-                            for(u32 j = M; j-- > 0u;)
-                                if(segPending[j] != 0u)
-                                    sj = j;
-                            if(sj == M)
-                                break;
-                            u64 spend = 0u;
-                            u32 sidx = 0u;
-#pragma unroll
-                            for(u32 j = 0; j < M; ++j)
-                                if(j == sj)
-                                {
-                                    spend = segPending[j];
-                                    sidx = idx[j];
-                                }
-                            const u32 sl = hw::ffs(spend) - 1u;
-                            const u32 seg = wave::lane(sidx, sl) / 32u;
-                            u32 segIn = 0u;
-#pragma unroll
-                            for(u32 j = 0; j < M; ++j)
+                            for(u64 segPending = group; segPending != 0u;)
                             {
-                                const bool inSeg = ((segPending[j] >> me) & 1u) != 0u && idx[j] / 32u == seg;
-                                if(inSeg)
-                                    segIn |= 1u << (idx[j] % 32u);
-                                segPending[j] &= ~hw::ballot(inSeg);
-                            }
-                            const u32 b = wave::bor(segIn);
-#pragma unroll
-                            for(u32 j = 0; j < M; ++j)
-                                if(j == sj && me == sl)
+                                const u32 sl = hw::ffs(segPending) - 1u;
+                                const u32 seg = wave::lane(idx[j], sl) / 32u;
+                                const bool inSeg = ((segPending >> me) & 1u) != 0u && idx[j] / 32u == seg;
+                                const u32 b = wave::bor(inSeg ? 1u << (idx[j] % 32u) : 0u);
+                                if(me == sl)
                                 {
                                     segLeader[j] = true;
                                     segBits[j] = b;
                                 }
+                                segPending &= ~hw::ballot(inSeg);
+                            }
                         }
+                        pending &= ~group;
                     }
-#pragma unroll
-                    for(u32 j = 0; j < M; ++j)
-                        pending[j] &= ~group[j];
                 }
 
                 // ---- clear the chunk bits and return the counts: every group at once
