@@ -128,25 +128,46 @@ namespace mallocMC
             return nullptr;
         }
 
+        //! The geometry constants that depend on the pagesize and the data
+        //! alignment only: the derived constants of Scatter.hpp:173-187 and
+        //! the side-mask width.  computeGeometry uses them, and so do the
+        //! compile-time copies of PinnedGeometry.
+        struct PageConstants
+        {
+            u32 hierarchyThreshold;
+            u32 minChunkSize;
+            u32 maxOnPageMasks;
+            u32 sideMaskWords;
+        };
+
+        MALLOCMC_HD constexpr PageConstants pageConstants(u32 P, u32 dataAlignment)
+        {
+            const u32 threshold = (P - 2u * 4u) / 33u;
+            const u32 minChunk = (P - 32u * 4u) / (32u * 32u);
+            const u32 minSegment = 32u * minChunk + 4u;
+            const u32 onPage = minChunk > threshold ? 0u : (P + (minSegment - 1u)) / minSegment;
+            // segments of the smallest chunk a page can hold bound the mask
+            // words any page needs (fewer for every larger chunk); padded
+            // requests are multiples of dataAlignment
+            const u32 align = dataAlignment - 1u;
+            const u32 padded = (minChunk + align) & ~align;
+            const u32 atLeast = dataAlignment > 1u ? dataAlignment : 1u;
+            const u32 cmin = padded > atLeast ? padded : atLeast;
+            const u32 side = P / (32u * cmin + 4u) + 1u;
+            return PageConstants{threshold, minChunk, onPage < 32u ? onPage : 32u, side < 32u ? side : 32u};
+        }
+
         //! Geometry as initDeviceFunction computes it (Scatter.hpp:962-1014) plus
         //! the derived constants of Scatter.hpp:173-187.
         inline ScatterGeometry computeGeometry(const ScatterConfig& c, u64 memsize)
         {
             ScatterGeometry g;
             const u32 P = c.pagesize;
-            g.hierarchyThreshold = (P - 2u * 4u) / 33u;
-            g.minChunkSize = (P - 32u * 4u) / (32u * 32u);
-            const u32 minSegment = 32u * g.minChunkSize + 4u;
-            const u32 onPage = g.minChunkSize > g.hierarchyThreshold ? 0u : (P + (minSegment - 1u)) / minSegment;
-            g.maxOnPageMasks = umin32(32u, onPage);
-            // segments of the smallest chunk a page can hold bound the mask
-            // words any page needs (fewer for every larger chunk); padded
-            // requests are multiples of dataAlignment
-            {
-                const u32 align = c.dataAlignment - 1u;
-                const u32 cmin = umax32((g.minChunkSize + align) & ~align, umax32(c.dataAlignment, 1u));
-                g.sideMaskWords = umin32(32u, P / (32u * cmin + 4u) + 1u);
-            }
+            const PageConstants pc = pageConstants(P, c.dataAlignment);
+            g.hierarchyThreshold = pc.hierarchyThreshold;
+            g.minChunkSize = pc.minChunkSize;
+            g.maxOnPageMasks = pc.maxOnPageMasks;
+            g.sideMaskWords = pc.sideMaskWords;
 
             const u64 R = c.regionsize;
             u32 numregions = static_cast<u32>(memsize / (R * (12u + static_cast<u64>(P)) + 4u));
@@ -838,5 +859,86 @@ namespace mallocMC
             h.divSlice8 = FastDiv(g.pagesperblock / 8u);
             return h;
         }
+
+        //! log2 of a power of two.
+        MALLOCMC_HD constexpr u32 log2Pow2(u32 x)
+        {
+            u32 s = 0u;
+            while((1u << s) < x)
+                ++s;
+            return s;
+        }
+
+        //! A heap view's configuration pinned at compile time: pagesize P and
+        //! regionsize R (powers of two), wastefactor 2, data alignment 16 and
+        //! the default hashing constants.  A kernel built for it overwrites
+        //! those fields of its copy of the view with constants, so that the
+        //! compiler folds them (divisions by R and P become shifts, thresholds
+        //! and hash factors become immediates) and keeps fewer scalar
+        //! registers live.  `matches` tells the launch whether a heap's view
+        //! holds exactly these values; the heap-size fields (page and region
+        //! counts, access blocks, pointers) and resetfreedpages and flags stay
+        //! as the view has them.
+        template<u32 P, u32 R>
+        struct PinnedGeometry
+        {
+            static_assert(P >= 128u && (P & (P - 1u)) == 0u && R != 0u && (R & (R - 1u)) == 0u,
+                          "pagesize and regionsize are powers of two");
+            static constexpr u32 kAlign = 16u;
+            static constexpr PageConstants kPage = pageConstants(P, kAlign);
+
+            MALLOCMC_HD static ScatterHeap apply(ScatterHeap h)
+            {
+                constexpr ScatterConfig kDefaults{};
+                h.pagesize = P;
+                h.pageShift = log2Pow2(P);
+                h.regionsize = R;
+                h.wastefactor = 2u;
+                h.hierarchyThreshold = kPage.hierarchyThreshold;
+                h.minChunkSize = kPage.minChunkSize;
+                h.maxOnPageMasks = kPage.maxOnPageMasks;
+                h.maskStride = kPage.sideMaskWords;
+                h.alignMask = kAlign - 1u;
+                h.paddedMinChunk = (kPage.minChunkSize + kAlign - 1u) & ~(kAlign - 1u);
+                h.hashingK = kDefaults.hashingK;
+                h.hashingDistMP = kDefaults.hashingDistMP;
+                h.hashingDistWP = kDefaults.hashingDistWP;
+                h.hashingDistWPRel = kDefaults.hashingDistWPRel;
+                h.divR.d = R;
+                h.divR.m = 0u;
+                h.divR.s = log2Pow2(R);
+                h.divP.d = P;
+                h.divP.m = 0u;
+                h.divP.s = log2Pow2(P);
+                return h;
+            }
+
+            static bool matches(const ScatterHeap& h)
+            {
+                const ScatterHeap p = apply(h);
+                return h.pagesize == p.pagesize && h.pageShift == p.pageShift && h.regionsize == p.regionsize
+                    && h.wastefactor == p.wastefactor && h.hierarchyThreshold == p.hierarchyThreshold
+                    && h.minChunkSize == p.minChunkSize && h.maxOnPageMasks == p.maxOnPageMasks
+                    && h.maskStride == p.maskStride && h.alignMask == p.alignMask
+                    && h.paddedMinChunk == p.paddedMinChunk && h.hashingK == p.hashingK
+                    && h.hashingDistMP == p.hashingDistMP && h.hashingDistWP == p.hashingDistWP
+                    && h.hashingDistWPRel == p.hashingDistWPRel && h.divR.d == p.divR.d && h.divR.m == p.divR.m
+                    && h.divR.s == p.divR.s && h.divP.d == p.divP.d && h.divP.m == p.divP.m && h.divP.s == p.divP.s;
+            }
+        };
+
+        //! No field pinned: the view as it is, for every configuration.
+        struct RuntimeGeometry
+        {
+            MALLOCMC_HD static ScatterHeap apply(const ScatterHeap& h)
+            {
+                return h;
+            }
+
+            static bool matches(const ScatterHeap&)
+            {
+                return true;
+            }
+        };
     } // namespace detail
 } // namespace mallocMC

@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = [
     "mmc_info", "mmc_available_slots", "mmc_available_slots_async", "mmc_get_handle",
     "mmc_malloc_batch", "mmc_free_batch", "mmc_serial_stream",
     "mmc_write_pattern", "mmc_verify_pattern", "mmc_alloc_until_null", "mmc_bench_atomics",
-    "mmc_bench_copy", "mmc_selftest_wave", "mmc_selftest_fastdiv",
+    "mmc_bench_copy", "mmc_selftest_wave", "mmc_selftest_fastdiv", "mmc_selftest_pinned_geometry",
 ]
 BATCHED_SYMBOLS = [
     "mmc_default_tuning", "mmc_wave_malloc_batch", "mmc_wave_free_batch", "mmc_multi_create",
@@ -240,6 +240,7 @@ def lib() -> ctypes.CDLL:
         "mmc_bench_copy": ([vp, vp, u64, vp], i32),
         "mmc_selftest_wave": ([u32, u64, vp, vp], i32),
         "mmc_selftest_fastdiv": ([u64, u64, P(u64)], i32),
+        "mmc_selftest_pinned_geometry": ([P(u32), P(u32)], i32),
         "mmc_default_tuning": ([P(MmcTuning)], i32),
         "mmc_wave_malloc_batch": ([vp, vp, vp, u64, P(MmcTuning), vp], i32),
         "mmc_wave_free_batch": ([vp, vp, u64, P(MmcTuning), vp], i32),

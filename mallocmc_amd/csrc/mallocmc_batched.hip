@@ -175,10 +175,13 @@ namespace
     //! by size bucket, and each wave serves 64 M of them through
     //! WaveAlloc::mallocMany, so that a size-class group spans up to 64 M
     //! requests and its scans and claims are paid once for all of them.
-    template<u32 B, u32 M>
-    __global__ __launch_bounds__(B) void waveMallocManyKernel(ScatterHeap h, WaveTuning t, const u32* sizes,
+    //! Geo: the heap configuration the kernel is built for (PinnedGeometry,
+    //! or RuntimeGeometry for any).
+    template<u32 B, u32 M, class Geo>
+    __global__ __launch_bounds__(B) void waveMallocManyKernel(ScatterHeap view, WaveTuning t, const u32* sizes,
                                                               u64* offsets, u64 n)
     {
+        const ScatterHeap h = Geo::apply(view);
         const u64 base = static_cast<u64>(blockIdx.x) * (B * M);
         u32 key[M], bytes[M], slot[M];
 #pragma unroll
@@ -220,9 +223,10 @@ namespace
     //! M frees per lane: the block's B x M frees are reordered by page
     //! index mod 256, and each wave frees 64 M of them through
     //! WaveAlloc::freeMany, so that more of a page's frees meet in one group.
-    template<u32 B, u32 M>
-    __global__ __launch_bounds__(B) void waveFreeManyKernel(ScatterHeap h, WaveTuning t, const u64* offsets, u64 n)
+    template<u32 B, u32 M, class Geo>
+    __global__ __launch_bounds__(B) void waveFreeManyKernel(ScatterHeap view, WaveTuning t, const u64* offsets, u64 n)
     {
+        const ScatterHeap h = Geo::apply(view);
         const u64 base = static_cast<u64>(blockIdx.x) * (B * M);
         u32 key[M], slot[M];
         u64 off[M];
@@ -240,6 +244,40 @@ namespace
         for(u32 j = 0; j < M; ++j)
             p[j] = off[j] == kNullOffset ? nullptr : h.pages + off[j];
         wa.freeMany<M>(p);
+    }
+
+    //! The configurations with kernels of their own (PinnedGeometry): the
+    //! pagesizes and regionsizes that lead the one-heap tuning space, with
+    //! the default wastefactor, alignment and hashing.
+    template<class... G>
+    struct GeometryList
+    {
+    };
+    using PinnedGeometries = GeometryList<PinnedGeometry<65536, 32>, PinnedGeometry<65536, 16>,
+                                          PinnedGeometry<16384, 16>, PinnedGeometry<16384, 32>>;
+
+    //! How many geometries of the list accept the view.
+    template<class... G>
+    u32 countMatches(const ScatterHeap& v, GeometryList<G...>)
+    {
+        return (0u + ... + (G::matches(v) ? 1u : 0u));
+    }
+
+    //! Calls launch(G{}) for the first geometry of the list that the view
+    //! matches, or launch(RuntimeGeometry{}).
+    template<class F>
+    void withGeometry(const ScatterHeap&, F&& launch, GeometryList<>)
+    {
+        launch(RuntimeGeometry{});
+    }
+
+    template<class F, class G0, class... Gs>
+    void withGeometry(const ScatterHeap& v, F&& launch, GeometryList<G0, Gs...>)
+    {
+        if(G0::matches(v))
+            launch(G0{});
+        else
+            withGeometry(v, launch, GeometryList<Gs...>{});
     }
 
     //! Work-queue helpers shared by the persistent kernels: cursor[0] hands out
@@ -810,6 +848,48 @@ namespace mmc
 
 extern "C" {
 
+int mmc_selftest_pinned_geometry(uint32_t* pinned, uint32_t* wrong)
+{
+    if(!pinned || !wrong)
+        return mmc::fail(MMC_EINVAL, "null output");
+    auto view = [](const ScatterConfig& c) {
+        const ScatterGeometry g = computeGeometry(c, 1ull << 30);
+        return makeHeapView(c, g, nullptr, 1ull << 30, nullptr, nullptr);
+    };
+    const u32 sizes[2][2] = {{65536u, 32u}, {16384u, 16u}};
+    const u32 more[2][2] = {{65536u, 16u}, {16384u, 32u}};
+    u32 ok = 0u, bad = 0u;
+    for(const auto* list : {sizes, more})
+        for(u32 k = 0; k < 2u; ++k)
+        {
+            ScatterConfig c;
+            c.pagesize = list[k][0];
+            c.regionsize = list[k][1];
+            c.resetfreedpages = 1u;
+            // the configuration itself: exactly one pinned geometry
+            ok += countMatches(view(c), PinnedGeometries{}) == 1u ? 1u : 0u;
+            // one field changed: none
+            ScatterConfig w = c;
+            w.wastefactor = 3u;
+            bad += countMatches(view(w), PinnedGeometries{});
+            w = c;
+            w.dataAlignment = 32u;
+            bad += countMatches(view(w), PinnedGeometries{});
+            w = c;
+            w.hashingK += 1u;
+            bad += countMatches(view(w), PinnedGeometries{});
+            w = c;
+            w.regionsize = 8u;
+            bad += countMatches(view(w), PinnedGeometries{});
+            w = c;
+            w.pagesize = 4096u;
+            bad += countMatches(view(w), PinnedGeometries{});
+        }
+    *pinned = ok;
+    *wrong = bad;
+    return MMC_OK;
+}
+
 int mmc_default_tuning(mmc_tuning* t)
 {
     if(!t)
@@ -860,28 +940,33 @@ int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_o
         {
             const unsigned blocks4 = mmc::gridFor(n, 4u * bs);
             if(bs == 1024u)
-                hipLaunchKernelGGL((waveMallocManyKernel<1024, 4>), dim3(blocks4), dim3(1024), 0, s, heap->view, t,
-                                   d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<1024, 4, RuntimeGeometry>), dim3(blocks4), dim3(1024), 0, s,
+                                   heap->view, t, d_sizes, d_offsets, n);
             else if(bs == 512u)
-                hipLaunchKernelGGL((waveMallocManyKernel<512, 4>), dim3(blocks4), dim3(512), 0, s, heap->view, t,
-                                   d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<512, 4, RuntimeGeometry>), dim3(blocks4), dim3(512), 0, s,
+                                   heap->view, t, d_sizes, d_offsets, n);
             else
-                hipLaunchKernelGGL((waveMallocManyKernel<256, 4>), dim3(blocks4), dim3(256), 0, s, heap->view, t,
-                                   d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<256, 4, RuntimeGeometry>), dim3(blocks4), dim3(256), 0, s,
+                                   heap->view, t, d_sizes, d_offsets, n);
         }
         else if(!xm && part != 0u && tuning && tuning->lane_requests >= 2u)
         {
             // two requests per lane: a block serves 2 x bs of them
             const unsigned blocks2 = mmc::gridFor(n, 2u * bs);
             if(bs == 1024u)
-                hipLaunchKernelGGL((waveMallocManyKernel<1024, 2>), dim3(blocks2), dim3(1024), 0, s, heap->view, t,
-                                   d_sizes, d_offsets, n);
+                withGeometry(
+                    heap->view,
+                    [&](auto g) {
+                        hipLaunchKernelGGL((waveMallocManyKernel<1024, 2, decltype(g)>), dim3(blocks2), dim3(1024), 0,
+                                           s, heap->view, t, d_sizes, d_offsets, n);
+                    },
+                    PinnedGeometries{});
             else if(bs == 512u)
-                hipLaunchKernelGGL((waveMallocManyKernel<512, 2>), dim3(blocks2), dim3(512), 0, s, heap->view, t,
-                                   d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<512, 2, RuntimeGeometry>), dim3(blocks2), dim3(512), 0, s,
+                                   heap->view, t, d_sizes, d_offsets, n);
             else
-                hipLaunchKernelGGL((waveMallocManyKernel<256, 2>), dim3(blocks2), dim3(256), 0, s, heap->view, t,
-                                   d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<256, 2, RuntimeGeometry>), dim3(blocks2), dim3(256), 0, s,
+                                   heap->view, t, d_sizes, d_offsets, n);
         }
         else if(xm)
         {
@@ -931,14 +1016,19 @@ int mmc_wave_free_batch(mmc_heap* heap, const uint64_t* d_offsets, uint64_t n, c
         // slower: a launch of one workgroup per CU waits for its slowest wave)
         const bool two = tuning && partitionOf(tuning) && tuning->lane_requests >= 2u;
         if(two && bs == 1024u)
-            hipLaunchKernelGGL((waveFreeManyKernel<1024, 2>), dim3(mmc::gridFor(n, 2048)), dim3(1024), 0, s,
-                               heap->view, t, d_offsets, n);
+            withGeometry(
+                heap->view,
+                [&](auto g) {
+                    hipLaunchKernelGGL((waveFreeManyKernel<1024, 2, decltype(g)>), dim3(mmc::gridFor(n, 2048)),
+                                       dim3(1024), 0, s, heap->view, t, d_offsets, n);
+                },
+                PinnedGeometries{});
         else if(two && bs == 512u)
-            hipLaunchKernelGGL((waveFreeManyKernel<512, 2>), dim3(mmc::gridFor(n, 1024)), dim3(512), 0, s,
-                               heap->view, t, d_offsets, n);
+            hipLaunchKernelGGL((waveFreeManyKernel<512, 2, RuntimeGeometry>), dim3(mmc::gridFor(n, 1024)), dim3(512), 0,
+                               s, heap->view, t, d_offsets, n);
         else if(two)
-            hipLaunchKernelGGL((waveFreeManyKernel<256, 2>), dim3(mmc::gridFor(n, 512)), dim3(256), 0, s,
-                               heap->view, t, d_offsets, n);
+            hipLaunchKernelGGL((waveFreeManyKernel<256, 2, RuntimeGeometry>), dim3(mmc::gridFor(n, 512)), dim3(256), 0,
+                               s, heap->view, t, d_offsets, n);
         else if(bs == 1024u)
             hipLaunchKernelGGL(waveFreeKernel<1024>, dim3(mmc::gridFor(n, 1024)), dim3(1024), 0, s, heap->view, t,
                                d_offsets, n, partitionOf(tuning));

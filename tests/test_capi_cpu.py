@@ -154,6 +154,15 @@ def test_serve_trace_shape():
     assert abs(len(t.arrival) / 30.0 - 2000) < 100  # Poisson 2000/s over 30 s
 
 
+def test_pinned_geometries_accept_their_heaps_only(mmc):
+    """The batch kernels built for fixed configurations run only on heaps
+    whose view holds exactly their compile-time constants."""
+    pinned, wrong = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    assert mmc.lib().mmc_selftest_pinned_geometry(ctypes.byref(pinned), ctypes.byref(wrong)) == 0
+    assert pinned.value == 4
+    assert wrong.value == 0
+
+
 def test_fastdiv_matches_division(mmc):
     """The heap view's division by invariant divisors (multiply-high and
     shifts) equals `/` and `%` on edge cases and 2,000,000 seeded pairs

@@ -64,16 +64,19 @@ def test_wave_16B(mmc, cuda, tk):
 
 
 @pytest.mark.parametrize("tk", range(len(TUNINGS)))
-@pytest.mark.parametrize("pagesize", [16384, 65536])
-def test_wave_mixed_sizes(mmc, cuda, tk, pagesize):
+@pytest.mark.parametrize("pagesize,regionsize", [(16384, 16), (65536, 16), (65536, 32), (16384, 32), (32768, 32)])
+def test_wave_mixed_sizes(mmc, cuda, tk, pagesize, regionsize):
+    """Every tuning on the configurations the two-requests-per-lane kernels
+    are built for at compile time (pagesize 16 or 64 KiB, regionsize 16 or
+    32) and on one they are not (32 KiB)."""
     torch = cuda
     t = mmc.Tuning(**TUNINGS[tk])
-    cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=2 * GiB, regionsize=16, resetfreedpages=True)
+    cfg = mmc.ScatterConfig(pagesize=pagesize, accessblocksize=2 * GiB, regionsize=regionsize, resetfreedpages=True)
     a = mmc.ScatterAllocator(cfg, 1 * GiB)
     s0 = [a.getAvailableSlots(s) for s in (16, 1000, 4096)]
     n = 262144
     for seed in (1, 2):
-        print(f"wave mixed P={pagesize} tuning={t} seed={seed}", flush=True)
+        print(f"wave mixed P={pagesize} R={regionsize} tuning={t} seed={seed}", flush=True)
         sizes = torch.from_numpy(wl.mixed_sizes(seed, n).astype(np.int32)).cuda()
         offs = a.wave_malloc_batch(sizes, tuning=t)
         a.write_pattern(offs, sizes, salt=seed)
