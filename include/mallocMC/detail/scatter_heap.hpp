@@ -940,5 +940,39 @@ namespace mallocMC
                 return true;
             }
         };
+
+        //! The configurations with kernels of their own (PinnedGeometry): the
+        //! pagesizes and regionsizes that lead the one-heap tuning space, with
+        //! the default wastefactor, alignment and hashing.
+        template<class... G>
+        struct GeometryList
+        {
+        };
+        using PinnedGeometries = GeometryList<PinnedGeometry<65536, 32>, PinnedGeometry<65536, 16>,
+                                              PinnedGeometry<16384, 16>, PinnedGeometry<16384, 32>>;
+
+        //! How many geometries of the list accept the view.
+        template<class... G>
+        u32 countMatches(const ScatterHeap& v, GeometryList<G...>)
+        {
+            return (0u + ... + (G::matches(v) ? 1u : 0u));
+        }
+
+        //! Calls launch(G{}) for the first geometry of the list that the view
+        //! matches, or launch(RuntimeGeometry{}).
+        template<class F>
+        void withGeometry(const ScatterHeap&, F&& launch, GeometryList<>)
+        {
+            launch(RuntimeGeometry{});
+        }
+
+        template<class F, class G0, class... Gs>
+        void withGeometry(const ScatterHeap& v, F&& launch, GeometryList<G0, Gs...>)
+        {
+            if(G0::matches(v))
+                launch(G0{});
+            else
+                withGeometry(v, launch, GeometryList<Gs...>{});
+        }
     } // namespace detail
 } // namespace mallocMC

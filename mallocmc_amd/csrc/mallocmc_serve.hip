@@ -173,8 +173,13 @@ namespace
     //! of 47 us).
     constexpr u32 kDecodeThreads = 64u;
 
-    __global__ __launch_bounds__(kDecodeThreads) void decodeKernel(ServeView v)
+    //! Geo: the heap configuration the kernel is built for (PinnedGeometry,
+    //! or RuntimeGeometry for any).
+    template<class Geo>
+    __global__ __launch_bounds__(kDecodeThreads) void decodeKernel(ServeView view)
     {
+        ServeView v = view;
+        v.heap = Geo::apply(view.heap);
         // the step's start stamp: the first workgroup, before any work
         if(v.stamps && blockIdx.x == 0u && threadIdx.x == 0u)
             stamp(v.sc, v.stamps, v.stampCapacity);
@@ -606,8 +611,13 @@ int mmc_serve_step(mmc_serve* s, void* stream)
     if(!s)
         return mmc::fail(MMC_EINVAL, "null argument");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(decodeKernel, dim3(mmc::gridFor(s->cfg.max_slots, kDecodeThreads)), dim3(kDecodeThreads), 0, st,
-                       s->view);
+    withGeometry(
+        s->view.heap,
+        [&](auto g) {
+            hipLaunchKernelGGL((decodeKernel<decltype(g)>), dim3(mmc::gridFor(s->cfg.max_slots, kDecodeThreads)),
+                               dim3(kDecodeThreads), 0, st, s->view);
+        },
+        PinnedGeometries{});
     hipLaunchKernelGGL(completeKernel, dim3(kCompleteBlocks + 1u), dim3(256), 0, st, s->view);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve step launch");
