@@ -75,6 +75,40 @@ namespace mallocMC
             u32 freshFast = 1u;
         };
 
+        //! The default wave tuning pinned at compile time: a kernel built
+        //! for it folds the tuning's branches and constants (the group cap,
+        //! the claim forms, no XCD slices) instead of keeping them in
+        //! registers; `matches` tells the launch whether a tuning is the
+        //! default.
+        struct DefaultTuning
+        {
+            MALLOCMC_HD static WaveTuning apply(const WaveTuning&)
+            {
+                return WaveTuning{};
+            }
+
+            static bool matches(const WaveTuning& t)
+            {
+                const WaveTuning d{};
+                return t.aggregation == d.aggregation && t.xcdSlices == d.xcdSlices && t.classSplit == d.classSplit
+                    && t.flatFast == d.flatFast && t.fastForward == d.fastForward && t.freshFast == d.freshFast;
+            }
+        };
+
+        //! Any tuning, read at run time.
+        struct RuntimeTuning
+        {
+            MALLOCMC_HD static WaveTuning apply(const WaveTuning& t)
+            {
+                return t;
+            }
+
+            static bool matches(const WaveTuning&)
+            {
+                return true;
+            }
+        };
+
         //! Wave-wide reductions and scans.  All 64 lanes must be active.
         //! Reductions use DPP (data-parallel lane moves inside the VALU: quad
         //! butterflies, row rotates and the GFX9 row broadcasts), not LDS

@@ -62,15 +62,16 @@ int mmc_selftest_wave(uint32_t waves, uint64_t seed, uint32_t* d_result, void* s
  * number of mismatches in *mismatches.  Runs on the host only. */
 int mmc_selftest_fastdiv(uint64_t seed, uint64_t count, uint64_t* mismatches);
 
-/* Host self-test of the heap configurations the batch kernels are built for
- * at compile time (pagesize 64 KiB or 16 KiB, regionsize 16 or 32,
- * wastefactor 2, alignment 16, default hashing), no reference counterpart:
- * *pinned receives how many of these configurations' heap views their
- * kernels accept (all four when the compile-time constants equal the ones a
- * heap computes), *wrong how many views of other configurations (another
- * wastefactor, alignment, hashing constant, pagesize or regionsize) one of
- * them accepts (none: those heaps run the kernels that read every constant
- * from the view).  Runs on the host only. */
+/* Host self-test of the heap configurations and the tuning the batch kernels
+ * are built for at compile time (pagesize 64 KiB or 16 KiB, regionsize 16 or
+ * 32, wastefactor 2, alignment 16, default hashing; mmc_default_tuning), no
+ * reference counterpart: *pinned receives how many of the four
+ * configurations' heap views and the default tuning their kernels accept
+ * (all five when the compile-time constants equal the ones a heap and
+ * mmc_default_tuning give), *wrong how many views of other configurations
+ * (another wastefactor, alignment, hashing constant, pagesize or regionsize)
+ * and other tunings (one field changed) they accept (none: those run the
+ * kernels that read every constant at run time).  Runs on the host only. */
 int mmc_selftest_pinned_geometry(uint32_t* pinned, uint32_t* wrong);
 
 #ifdef __cplusplus

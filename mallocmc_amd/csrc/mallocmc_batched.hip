@@ -9,6 +9,7 @@
 #include "mmc_internal.hpp"
 
 #include <mutex>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -175,13 +176,15 @@ namespace
     //! by size bucket, and each wave serves 64 M of them through
     //! WaveAlloc::mallocMany, so that a size-class group spans up to 64 M
     //! requests and its scans and claims are paid once for all of them.
-    //! Geo: the heap configuration the kernel is built for (PinnedGeometry,
-    //! or RuntimeGeometry for any).
-    template<u32 B, u32 M, class Geo>
-    __global__ __launch_bounds__(B) void waveMallocManyKernel(ScatterHeap view, WaveTuning t, const u32* sizes,
+    //! Geo and Tun: the heap configuration and the tuning the kernel is
+    //! built for (PinnedGeometry and DefaultTuning, or RuntimeGeometry and
+    //! RuntimeTuning for any).
+    template<u32 B, u32 M, class Geo, class Tun>
+    __global__ __launch_bounds__(B) void waveMallocManyKernel(ScatterHeap view, WaveTuning tuning, const u32* sizes,
                                                               u64* offsets, u64 n)
     {
         const ScatterHeap h = Geo::apply(view);
+        const WaveTuning t = Tun::apply(tuning);
         const u64 base = static_cast<u64>(blockIdx.x) * (B * M);
         u32 key[M], bytes[M], slot[M];
 #pragma unroll
@@ -851,6 +854,27 @@ int mmc_selftest_pinned_geometry(uint32_t* pinned, uint32_t* wrong)
             w.pagesize = 4096u;
             bad += countMatches(view(w), PinnedGeometries{});
         }
+    // the default tuning: the kernels built for it run for mmc_default_tuning only
+    mmc_tuning d;
+    mmc_default_tuning(&d);
+    ok += DefaultTuning::matches(toTuning(&d)) ? 1u : 0u;
+    for(u32 k = 0; k < 6u; ++k)
+    {
+        mmc_tuning w = d;
+        if(k == 0u)
+            w.aggregation = 16u;
+        else if(k == 1u)
+            w.xcd_slices = 8u;
+        else if(k == 2u)
+            w.class_split = 1u;
+        else if(k == 3u)
+            w.flat_single = 1u;
+        else if(k == 4u)
+            w.no_skip = 1u;
+        else
+            w.fresh_single = 1u;
+        bad += DefaultTuning::matches(toTuning(&w)) ? 1u : 0u;
+    }
     *pinned = ok;
     *wrong = bad;
     return MMC_OK;
@@ -906,14 +930,14 @@ int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_o
         {
             const unsigned blocks4 = mmc::gridFor(n, 4u * bs);
             if(bs == 1024u)
-                hipLaunchKernelGGL((waveMallocManyKernel<1024, 4, RuntimeGeometry>), dim3(blocks4), dim3(1024), 0, s,
-                                   heap->view, t, d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<1024, 4, RuntimeGeometry, RuntimeTuning>), dim3(blocks4),
+                                   dim3(1024), 0, s, heap->view, t, d_sizes, d_offsets, n);
             else if(bs == 512u)
-                hipLaunchKernelGGL((waveMallocManyKernel<512, 4, RuntimeGeometry>), dim3(blocks4), dim3(512), 0, s,
-                                   heap->view, t, d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<512, 4, RuntimeGeometry, RuntimeTuning>), dim3(blocks4),
+                                   dim3(512), 0, s, heap->view, t, d_sizes, d_offsets, n);
             else
-                hipLaunchKernelGGL((waveMallocManyKernel<256, 4, RuntimeGeometry>), dim3(blocks4), dim3(256), 0, s,
-                                   heap->view, t, d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<256, 4, RuntimeGeometry, RuntimeTuning>), dim3(blocks4),
+                                   dim3(256), 0, s, heap->view, t, d_sizes, d_offsets, n);
         }
         else if(!xm && part != 0u && tuning && tuning->lane_requests >= 2u)
         {
@@ -923,16 +947,27 @@ int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_o
                 withGeometry(
                     heap->view,
                     [&](auto g) {
-                        hipLaunchKernelGGL((waveMallocManyKernel<1024, 2, decltype(g)>), dim3(blocks2), dim3(1024), 0,
-                                           s, heap->view, t, d_sizes, d_offsets, n);
+                        using G = decltype(g);
+                        // the default tuning is pinned only beside a pinned geometry
+                        if constexpr(!std::is_same_v<G, RuntimeGeometry>)
+                        {
+                            if(DefaultTuning::matches(t))
+                            {
+                                hipLaunchKernelGGL((waveMallocManyKernel<1024, 2, G, DefaultTuning>), dim3(blocks2),
+                                                   dim3(1024), 0, s, heap->view, t, d_sizes, d_offsets, n);
+                                return;
+                            }
+                        }
+                        hipLaunchKernelGGL((waveMallocManyKernel<1024, 2, G, RuntimeTuning>), dim3(blocks2),
+                                           dim3(1024), 0, s, heap->view, t, d_sizes, d_offsets, n);
                     },
                     PinnedGeometries{});
             else if(bs == 512u)
-                hipLaunchKernelGGL((waveMallocManyKernel<512, 2, RuntimeGeometry>), dim3(blocks2), dim3(512), 0, s,
-                                   heap->view, t, d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<512, 2, RuntimeGeometry, RuntimeTuning>), dim3(blocks2),
+                                   dim3(512), 0, s, heap->view, t, d_sizes, d_offsets, n);
             else
-                hipLaunchKernelGGL((waveMallocManyKernel<256, 2, RuntimeGeometry>), dim3(blocks2), dim3(256), 0, s,
-                                   heap->view, t, d_sizes, d_offsets, n);
+                hipLaunchKernelGGL((waveMallocManyKernel<256, 2, RuntimeGeometry, RuntimeTuning>), dim3(blocks2),
+                                   dim3(256), 0, s, heap->view, t, d_sizes, d_offsets, n);
         }
         else if(xm)
         {

@@ -155,11 +155,12 @@ def test_serve_trace_shape():
 
 
 def test_pinned_geometries_accept_their_heaps_only(mmc):
-    """The batch kernels built for fixed configurations run only on heaps
-    whose view holds exactly their compile-time constants."""
+    """The batch kernels built for fixed configurations (and for the default
+    tuning) run only on heaps whose view holds exactly their compile-time
+    constants, and only with the default tuning."""
     pinned, wrong = ctypes.c_uint32(0), ctypes.c_uint32(0)
     assert mmc.lib().mmc_selftest_pinned_geometry(ctypes.byref(pinned), ctypes.byref(wrong)) == 0
-    assert pinned.value == 4
+    assert pinned.value == 5
     assert wrong.value == 0
 
 

@@ -37,6 +37,7 @@ TUNINGS = [
     dict(aggregation=32, xcd_slices=0, persistent=False, block_threads=512),
     dict(aggregation=64, xcd_slices=0, persistent=False, fresh_single=True),
     dict(aggregation=64, xcd_slices=8, persistent=False, lane_requests=2),
+    dict(aggregation=64, xcd_slices=0, persistent=False, lane_requests=2),  # the default tuning
     dict(aggregation=64, xcd_slices=0, persistent=False, lane_requests=4),
     dict(aggregation=16, xcd_slices=0, persistent=False, lane_requests=2, block_threads=512),
     dict(aggregation=64, xcd_slices=8, persistent=False, lane_requests=4, block_threads=256, fresh_single=True),
