@@ -327,8 +327,11 @@ namespace
     //! reads.  Workgroups without a chunk (decode added up the step's chunk
     //! count) leave at once and touch no shared word: a ticket taken by every
     //! workgroup of the grid, on one address, cost about 7 us of the step.
-    __global__ __launch_bounds__(256) void completeKernel(ServeView v)
+    template<class Geo>
+    __global__ __launch_bounds__(256) void completeKernel(ServeView view)
     {
+        ServeView v = view;
+        v.heap = Geo::apply(view.heap);
         __shared__ u64 partial[4];
         const u32 lane = hw::laneid();
         const u32 q = threadIdx.x / 64u;
@@ -616,9 +619,9 @@ int mmc_serve_step(mmc_serve* s, void* stream)
         [&](auto g) {
             hipLaunchKernelGGL((decodeKernel<decltype(g)>), dim3(mmc::gridFor(s->cfg.max_slots, kDecodeThreads)),
                                dim3(kDecodeThreads), 0, st, s->view);
+            hipLaunchKernelGGL((completeKernel<decltype(g)>), dim3(kCompleteBlocks + 1u), dim3(256), 0, st, s->view);
         },
         PinnedGeometries{});
-    hipLaunchKernelGGL(completeKernel, dim3(kCompleteBlocks + 1u), dim3(256), 0, st, s->view);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "serve step launch");
 }
