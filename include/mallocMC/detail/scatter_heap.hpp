@@ -951,6 +951,13 @@ namespace mallocMC
         using PinnedGeometries = GeometryList<PinnedGeometry<65536, 32>, PinnedGeometry<65536, 16>,
                                               PinnedGeometry<16384, 16>, PinnedGeometry<16384, 32>>;
 
+        //! The length of the list.
+        template<class... G>
+        constexpr u32 geometryCount(GeometryList<G...>)
+        {
+            return static_cast<u32>(sizeof...(G));
+        }
+
         //! How many geometries of the list accept the view.
         template<class... G>
         u32 countMatches(const ScatterHeap& v, GeometryList<G...>)
@@ -973,6 +980,39 @@ namespace mallocMC
                 launch(G0{});
             else
                 withGeometry(v, launch, GeometryList<Gs...>{});
+        }
+
+        //! Position of the first geometry of the list that every one of the
+        //! n views matches; the list's length if none does.
+        template<class... G>
+        u32 commonGeometry(const ScatterHeap* views, u32 n, GeometryList<G...>)
+        {
+            auto all = [views, n](auto g) {
+                for(u32 i = 0; i < n; ++i)
+                    if(!decltype(g)::matches(views[i]))
+                        return false;
+                return true;
+            };
+            u32 k = 0u;
+            const bool found = ((all(G{}) ? true : (++k, false)) || ...);
+            return found ? k : static_cast<u32>(sizeof...(G));
+        }
+
+        //! Calls launch(G{}) for geometry `index` of the list, or
+        //! launch(RuntimeGeometry{}) for an index past its end.
+        template<class F>
+        void withGeometryAt(u32, F&& launch, GeometryList<>)
+        {
+            launch(RuntimeGeometry{});
+        }
+
+        template<class F, class G0, class... Gs>
+        void withGeometryAt(u32 index, F&& launch, GeometryList<G0, Gs...>)
+        {
+            if(index == 0u)
+                launch(G0{});
+            else
+                withGeometryAt(index - 1u, launch, GeometryList<Gs...>{});
         }
     } // namespace detail
 } // namespace mallocMC

@@ -4,6 +4,7 @@ Usage: python -m mallocmc_amd.build [--verbose]
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -50,14 +51,17 @@ def build(verbose: bool = False) -> str:
     cc = hipcc()
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with tempfile.TemporaryDirectory(prefix="mmc_build_") as tmp:
-        objs = []
-        for s in srcs:
-            obj = os.path.join(tmp, s.replace(".hip", ".o"))
-            cmd = [cc, *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, s), "-o", obj]
+        objs = [os.path.join(tmp, s.replace(".hip", ".o")) for s in srcs]
+        cmds = [[cc, *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, s), "-o", obj]
+                for s, obj in zip(srcs, objs)]
+        for cmd in cmds:
             if verbose:
                 print(" ".join(cmd), flush=True)
-            subprocess.run(cmd, check=True)
-            objs.append(obj)
+        # one hipcc per source, at most four at once (the batch kernels'
+        # file takes most of the time)
+        with concurrent.futures.ThreadPoolExecutor(max_workers=4) as pool:
+            for r in pool.map(lambda c: subprocess.run(c, check=True), cmds):
+                pass
         out_tmp = LIB_PATH + ".tmp"
         cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", out_tmp]
         if verbose:

@@ -21,6 +21,9 @@ struct mmc_multi
     ScatterHeap* dHeaps = nullptr;
     std::uint32_t nheaps = 0;
     std::uint32_t distribution = MMC_DIST_NOOP;
+    //! position in PinnedGeometries of the configuration every heap has
+    //! (their count if they have none of them in common)
+    std::uint32_t geometry = 0;
     int device = 0;
 };
 
@@ -429,7 +432,9 @@ namespace
     //! mallocs+frees/s at 1,024 threads; 512 threads: 1.26e9 against 0.94e9
     //! at the compiler's own 82 VGPRs), the spill of about 27 registers
     //! notwithstanding.
-    template<class Dist, u32 B>
+    //! Geo: the heap configuration the kernel is built for (PinnedGeometry,
+    //! when every heap of the step has it, or RuntimeGeometry for any).
+    template<class Dist, u32 B, class Geo>
     __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) void multiStepBlockKernel(
         const ScatterHeap* heaps, mmc_step_desc d, WaveTuning t, bool partition)
     {
@@ -481,7 +486,7 @@ namespace
         // the view is the same for the whole wave: keep it in scalar registers
         // (a workgroup of padding only reads heap 0 and does nothing with it)
         const u32 viewIdx = haveTile ? heapIdx : (blockHeap != 0xffffffffu ? blockHeap : 0u);
-        const ScatterHeap h = uniformCopy(heaps[__builtin_amdgcn_readfirstlane(viewIdx)]);
+        const ScatterHeap h = Geo::apply(uniformCopy(heaps[__builtin_amdgcn_readfirstlane(viewIdx)]));
         const WaveAlloc wa{h, t};
         const u64 r = mix64(d.seed ^ mix64(step * 0x100000001B3ull + s));
         const u32 coin = static_cast<u32>(r % 1000u);
@@ -698,17 +703,28 @@ namespace
         }
     }
 
+    //! `geometry`: the position in PinnedGeometries of the configuration
+    //! every heap of the table has (kernels built for it run with 1,024-thread
+    //! workgroups and the Noop distribution), or the list's length.
     template<u32 B>
     void launchMultiBlock(bool xm, const ScatterHeap* heaps, const mmc_step_desc& d, const WaveTuning& t,
-                          bool part, hipStream_t s)
+                          bool part, u32 geometry, hipStream_t s)
     {
         const unsigned blocks = (d.n_tiles + B / 64u - 1u) / (B / 64u);
         if(xm)
-            hipLaunchKernelGGL((multiStepBlockKernel<XMallocSIMD64, B>), dim3(blocks), dim3(B), 0, s, heaps, d, t,
-                               part);
+            hipLaunchKernelGGL((multiStepBlockKernel<XMallocSIMD64, B, RuntimeGeometry>), dim3(blocks), dim3(B), 0,
+                               s, heaps, d, t, part);
+        else if constexpr(B == 1024u)
+            withGeometryAt(
+                geometry,
+                [&](auto g) {
+                    hipLaunchKernelGGL((multiStepBlockKernel<NoopDistribution, B, decltype(g)>), dim3(blocks),
+                                       dim3(B), 0, s, heaps, d, t, part);
+                },
+                PinnedGeometries{});
         else
-            hipLaunchKernelGGL((multiStepBlockKernel<NoopDistribution, B>), dim3(blocks), dim3(B), 0, s, heaps, d,
-                               t, part);
+            hipLaunchKernelGGL((multiStepBlockKernel<NoopDistribution, B, RuntimeGeometry>), dim3(blocks), dim3(B),
+                               0, s, heaps, d, t, part);
     }
 
     unsigned persistentBlocks(const mmc_tuning* t)
@@ -728,7 +744,7 @@ namespace
     //! One multi-heap step over the device heap table `heaps`, in the form
     //! the tuning selects.
     void launchStep(const ScatterHeap* heaps, bool xm, const mmc_step_desc& d, const mmc_tuning* tuning,
-                    hipStream_t s)
+                    u32 geometry, hipStream_t s)
     {
         const WaveTuning t = toTuning(tuning);
         if(!tuning || tuning->persistent)
@@ -744,13 +760,13 @@ namespace
         switch(blockOf(tuning))
         {
         case 1024u:
-            launchMultiBlock<1024>(xm, heaps, d, t, part, s);
+            launchMultiBlock<1024>(xm, heaps, d, t, part, geometry, s);
             break;
         case 512u:
-            launchMultiBlock<512>(xm, heaps, d, t, part, s);
+            launchMultiBlock<512>(xm, heaps, d, t, part, geometry, s);
             break;
         default:
-            launchMultiBlock<256>(xm, heaps, d, t, part, s);
+            launchMultiBlock<256>(xm, heaps, d, t, part, geometry, s);
             break;
         }
     }
@@ -1087,6 +1103,7 @@ int mmc_multi_create(mmc_heap* const* heaps, uint32_t nheaps, mmc_multi** out)
     mmc_multi* m = new mmc_multi();
     m->nheaps = nheaps;
     m->distribution = dist;
+    m->geometry = commonGeometry(views.data(), nheaps, PinnedGeometries{});
     hipError_t err = hipMalloc(&m->dHeaps, sizeof(ScatterHeap) * nheaps);
     if(err == hipSuccess)
         err = hipMemcpy(m->dHeaps, views.data(), sizeof(ScatterHeap) * nheaps, hipMemcpyHostToDevice);
@@ -1118,7 +1135,8 @@ int mmc_multi_step(mmc_multi* m, const mmc_step_desc* desc, const mmc_tuning* tu
         return mmc::fail(MMC_EINVAL, "bad step descriptor");
     if(desc->n_tiles == 0)
         return MMC_OK;
-    launchStep(m->dHeaps, m->distribution == MMC_DIST_XMALLOCSIMD, *desc, tuning, static_cast<hipStream_t>(stream));
+    launchStep(m->dHeaps, m->distribution == MMC_DIST_XMALLOCSIMD, *desc, tuning, m->geometry,
+               static_cast<hipStream_t>(stream));
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "multi step launch");
 }
@@ -1157,6 +1175,13 @@ int mmc_multi_step_static(mmc_heap* const* heaps, uint32_t nheaps, const mmc_ste
     // the array keeps the last call's table: rewrite this call's views, and
     // start the tile cursor from zero (kernels, not memsets, so a capture
     // records kernel nodes only)
+    // the configuration every heap has, among those with kernels of their own
+    // (a view matches at most one of them: each pins another pagesize and
+    // regionsize)
+    u32 geometry = commonGeometry(&heaps[0]->view, 1u, PinnedGeometries{});
+    for(u32 i = 1; i < nheaps; ++i)
+        if(commonGeometry(&heaps[i]->view, 1u, PinnedGeometries{}) != geometry)
+            geometry = geometryCount(PinnedGeometries{});
     for(u32 first = 0; first < nheaps; first += kViewsPerLaunch)
     {
         ViewChunk c;
@@ -1168,7 +1193,7 @@ int mmc_multi_step_static(mmc_heap* const* heaps, uint32_t nheaps, const mmc_ste
     }
     mmc_step_desc d = *desc;
     d.d_tile_cursor = cursor;
-    launchStep(table, dist == MMC_DIST_XMALLOCSIMD, d, tuning, s);
+    launchStep(table, dist == MMC_DIST_XMALLOCSIMD, d, tuning, geometry, s);
     err = hipGetLastError();
     return err == hipSuccess ? MMC_OK : mmc::hipFail(err, "multi step launch");
 }
