@@ -33,7 +33,9 @@ def _run_all(loop, trace, chunk=512, max_steps=200000):
     return loop.stats()
 
 
-@pytest.mark.parametrize("pagesize,regionsize", [(16384, 16), (65536, 32)])
+# the first two have kernels built for their configuration, the third runs the
+# kernels that read every constant from the heap view
+@pytest.mark.parametrize("pagesize,regionsize", [(16384, 16), (65536, 32), (32768, 32)])
 def test_serve_checksums_and_restore(mmc, cuda, pagesize, regionsize):
     from mallocmc_amd import serve
     a = _alloc(mmc, pagesize=pagesize, regionsize=regionsize)
