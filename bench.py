@@ -291,6 +291,9 @@ def point_key(point):
 
 def kernel_names(point):
     """(malloc, free) kernels a one-heap batch step launches at `point`."""
+    if point.lane_requests == 2:
+        # 1,024-thread blocks (the bench's): sorted windows taken from a queue
+        return "waveMallocWindowsKernel", "waveFreeManyKernel"
     if point.lane_requests > 1:
         return "waveMallocManyKernel", "waveFreeManyKernel"
     return "waveMallocKernel", "waveFreeKernel"

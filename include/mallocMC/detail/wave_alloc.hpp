@@ -1462,25 +1462,16 @@ namespace mallocMC
                 return dist.distribute(r);
             }
 
-            //! Wave malloc of M requests per lane: slot j of lane l is the
-            //! wave's request 64 j + l.  For callers that hand a wave more
-            //! requests than it has lanes (the batch kernels): a size-class
-            //! group then spans up to 64 M requests, served by the same scans
-            //! and claims as malloc(), so the work a group pays once (its scan
-            //! windows, claim round trips and wave reductions) is spread over
-            //! more requests.  Groups are capped at aggregation x M requests.
-            //! There is no distribution policy: XMallocSIMD coalesces the 64
-            //! lanes of one call and keeps malloc().
+            //! One request set of mallocMany: pads the sizes, serves the
+            //! requests above a page (one wave-wide run claim each, as in
+            //! malloc(); a run of k pages that was not found ends the search
+            //! for runs of k or more, `failedPages`), and returns the other
+            //! requests as pending, with their size classes.
             template<u32 M>
-            MALLOCMC_D void mallocMany(const u32 (&bytesIn)[M], void* (&res)[M]) const
+            MALLOCMC_D void beginSet(const u32 (&bytesIn)[M], u32 (&b)[M], char* (&result)[M], u64 (&pending)[M],
+                                     u32 (&cls)[M], u32& failedPages) const
             {
                 const u32 me = hw::laneid();
-                // the access-block cursor, as in malloc()
-                u32 ffb = 0u;
-                if(me == 0u && h.accessblocks > 1u)
-                    ffb = atom::load(&h.scalars->firstfreeblock);
-                u32 b[M];
-                char* result[M];
                 bool want[M];
 #pragma unroll
                 for(u32 j = 0; j < M; ++j)
@@ -1489,10 +1480,6 @@ namespace mallocMC
                     result[j] = nullptr;
                     want[j] = b[j] != 0u;
                 }
-                // requests above a page: one wave-wide run claim each, as in
-                // malloc(); a run of k pages that was not found ends the
-                // search for runs of k or more
-                u32 failedPages = 0xffffffffu;
 #pragma unroll
                 for(u32 j = 0; j < M; ++j)
                 {
@@ -1514,87 +1501,174 @@ namespace mallocMC
                     if(b[j] > h.pagesize)
                         want[j] = false;
                 }
-                u64 pending[M];
-                u32 cls[M];
 #pragma unroll
                 for(u32 j = 0; j < M; ++j)
                 {
                     pending[j] = hw::ballot(want[j]);
                     cls[j] = want[j] ? sizeClass(h, b[j], t.classSplit) : 0xffffffffu;
                 }
-                ffb = wave::lane(ffb, 0u);
-                const u32 cap = t.aggregation * M;
-                for(;;)
-                {
-                    // the leader: the lowest pending request in (slot, lane) order
-                    u32 lj = M;
+            }
+
+            //! Serves one size-class group of the pending requests: the class
+            //! of the lowest pending request in (slot, lane) order (slot lj),
+            //! at most `cap` of its requests in that order.  Chunks go straight
+            //! to `result` (only served ranks are written); the group's
+            //! requests leave `pending`, served or not (not served: the heap
+            //! has no room for them).
+            template<u32 M>
+            MALLOCMC_D void serveGroup(u64 (&pending)[M], const u32 (&cls)[M], const u32 (&b)[M], char* (&result)[M],
+                                       u32 lj, u32 cap, u32 ffb) const
+            {
+                const u32 me = hw::laneid();
+                u64 lpend = 0u;
+                u32 lc = 0u;
 #pragma unroll
-                    for(u32 j = M; j-- > 0u;)
-                        if(pending[j] != 0u)
-                            lj = j;
-                    if(lj == M)
-                        break;
-                    u64 lpend = 0u;
-                    u32 lc = 0u;
-#pragma unroll
-                    for(u32 j = 0; j < M; ++j)
-                        if(j == lj)
-                        {
-                            lpend = pending[j];
-                            lc = cls[j];
-                        }
-                    const u32 lcls = wave::lane(lc, hw::ffs(lpend) - 1u);
-                    RankNeed<M> need;
-                    u64 group[M];
-                    u32 K = 0u;
-#pragma unroll
-                    for(u32 j = 0; j < M; ++j)
+                for(u32 j = 0; j < M; ++j)
+                    if(j == lj)
                     {
-                        const bool in = ((pending[j] >> me) & 1u) != 0u && cls[j] == lcls;
-                        group[j] = hw::ballot(in);
-                        need.rank.rank[j] = in ? K + hw::rankBelow(group[j]) : kNoRank;
-                        K += static_cast<u32>(hw::popc(group[j]));
+                        lpend = pending[j];
+                        lc = cls[j];
                     }
-                    if(K > cap)
-                    {
-                        // the lowest `cap` ranks form the group; the rest stay pending
-#pragma unroll
-                        for(u32 j = 0; j < M; ++j)
-                        {
-                            if(need.rank.rank[j] >= cap)
-                                need.rank.rank[j] = kNoRank;
-                            group[j] = hw::ballot(need.rank.rank[j] != kNoRank);
-                        }
-                        K = cap;
-                    }
-                    need.K = K;
-                    need.got = 0u;
-                    u32 bMax = 0u, bMin = 0xffffffffu;
-#pragma unroll
-                    for(u32 j = 0; j < M; ++j)
-                        if(need.rank.rank[j] != kNoRank)
-                        {
-                            bMax = umax32(bMax, b[j]);
-                            bMin = umin32(bMin, b[j]);
-                        }
-                    const u32 bmax = wave::max(bMax);
-                    const u32 c = umax32(bmax, h.paddedMinChunk); // the largest member's chunk
-                    const u32 bmin = wave::min(bMin);
-                    const u32 maxcs = umin32(h.pagesize, umax32(h.wastefactor * bmin, h.paddedMinChunk));
-                    // chunks go straight to `result`: only served ranks are written
-                    need = allocGroup<M>(need, c, bmax, maxcs, ffb, result);
-                    // requests of the group not served: the heap has no room for them
-#pragma unroll
-                    for(u32 j = 0; j < M; ++j)
-                        pending[j] &= ~group[j];
-                }
+                const u32 lcls = wave::lane(lc, hw::ffs(lpend) - 1u);
+                RankNeed<M> need;
+                u64 group[M];
+                u32 K = 0u;
 #pragma unroll
                 for(u32 j = 0; j < M; ++j)
                 {
-                    void* r = result[j];
-                    if(isOOM(r, b[j]))
-                        r = nullptr;
-                    res[j] = r;
+                    const bool in = ((pending[j] >> me) & 1u) != 0u && cls[j] == lcls;
+                    group[j] = hw::ballot(in);
+                    need.rank.rank[j] = in ? K + hw::rankBelow(group[j]) : kNoRank;
+                    K += static_cast<u32>(hw::popc(group[j]));
+                }
+                if(K > cap)
+                {
+                    // the lowest `cap` ranks form the group; the rest stay pending
+#pragma unroll
+                    for(u32 j = 0; j < M; ++j)
+                    {
+                        if(need.rank.rank[j] >= cap)
+                            need.rank.rank[j] = kNoRank;
+                        group[j] = hw::ballot(need.rank.rank[j] != kNoRank);
+                    }
+                    K = cap;
+                }
+                need.K = K;
+                need.got = 0u;
+                u32 bMax = 0u, bMin = 0xffffffffu;
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                    if(need.rank.rank[j] != kNoRank)
+                    {
+                        bMax = umax32(bMax, b[j]);
+                        bMin = umin32(bMin, b[j]);
+                    }
+                const u32 bmax = wave::max(bMax);
+                const u32 c = umax32(bmax, h.paddedMinChunk); // the largest member's chunk
+                const u32 bmin = wave::min(bMin);
+                const u32 maxcs = umin32(h.pagesize, umax32(h.wastefactor * bmin, h.paddedMinChunk));
+                // chunks go straight to `result`: only served ranks are written
+                allocGroup<M>(need, c, bmax, maxcs, ffb, result);
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                    pending[j] &= ~group[j];
+            }
+
+            //! The leader slot of the next group: the lowest slot with a
+            //! pending request, or M when none is left.
+            template<u32 M>
+            MALLOCMC_D static u32 leaderSlot(const u64 (&pending)[M])
+            {
+                u32 lj = M;
+#pragma unroll
+                for(u32 j = M; j-- > 0u;)
+                    if(pending[j] != 0u)
+                        lj = j;
+                return lj;
+            }
+
+            //! Wave malloc of M requests per lane: slot j of lane l is the
+            //! wave's request 64 j + l.  For callers that hand a wave more
+            //! requests than it has lanes (the batch kernels): a size-class
+            //! group then spans up to 64 M requests, served by the same scans
+            //! and claims as malloc(), so the work a group pays once (its scan
+            //! windows, claim round trips and wave reductions) is spread over
+            //! more requests.  Groups are capped at aggregation x M requests.
+            //! There is no distribution policy: XMallocSIMD coalesces the 64
+            //! lanes of one call and keeps malloc().
+            template<u32 M>
+            MALLOCMC_D void mallocMany(const u32 (&bytesIn)[M], void* (&res)[M]) const
+            {
+                const u32 me = hw::laneid();
+                // the access-block cursor, as in malloc()
+                u32 ffb = 0u;
+                if(me == 0u && h.accessblocks > 1u)
+                    ffb = atom::load(&h.scalars->firstfreeblock);
+                u32 b[M], cls[M];
+                char* result[M];
+                u64 pending[M];
+                u32 failedPages = 0xffffffffu;
+                beginSet<M>(bytesIn, b, result, pending, cls, failedPages);
+                ffb = wave::lane(ffb, 0u);
+                const u32 cap = t.aggregation * M;
+                for(u32 lj = leaderSlot<M>(pending); lj != M; lj = leaderSlot<M>(pending))
+                    serveGroup<M>(pending, cls, b, result, lj, cap, ffb);
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                    res[j] = isOOM(result[j], b[j]) ? nullptr : result[j];
+            }
+
+            //! mallocMany over a stream of request sets in one group loop:
+            //! `src.next(bytes)` fills the next set (false when there is none;
+            //! the same answer on every lane) and `src.done(res)` takes the
+            //! results of the set before.  A wave that serves several sets
+            //! keeps the registers of one loop: a loop of mallocMany calls
+            //! would keep every heap constant the call uses live across all of
+            //! it.
+            template<u32 M, class Src>
+            MALLOCMC_D void mallocStream(Src& src) const
+            {
+                const u32 me = hw::laneid();
+                u32 ffb = 0u;
+                if(me == 0u && h.accessblocks > 1u)
+                    ffb = atom::load(&h.scalars->firstfreeblock);
+                ffb = wave::lane(ffb, 0u);
+                const u32 cap = t.aggregation * M;
+                u32 failedPages = 0xffffffffu;
+                u32 b[M], cls[M];
+                char* result[M];
+                u64 pending[M];
+                bool open = false; // a set is being served
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    b[j] = 0u;
+                    cls[j] = 0xffffffffu;
+                    result[j] = nullptr;
+                    pending[j] = 0u;
+                }
+                for(;;)
+                {
+                    const u32 lj = leaderSlot<M>(pending);
+                    if(lj != M)
+                    {
+                        serveGroup<M>(pending, cls, b, result, lj, cap, ffb);
+                        continue;
+                    }
+                    // the set is served: hand over its results, take the next
+                    if(open)
+                    {
+                        void* r[M];
+#pragma unroll
+                        for(u32 j = 0; j < M; ++j)
+                            r[j] = isOOM(result[j], b[j]) ? nullptr : result[j];
+                        src.done(r);
+                    }
+                    u32 bytesIn[M];
+                    if(!src.next(bytesIn))
+                        break;
+                    open = true;
+                    beginSet<M>(bytesIn, b, result, pending, cls, failedPages);
                 }
             }
 

@@ -81,26 +81,23 @@ namespace
         return 223u - (8u * (e - 2u) + ((x >> (e - 3u)) & 7u));  // 0..215
     }
 
-    //! Reorders the B x M values of a block (B = 256, 512 or 1024 threads, M
-    //! values per thread) by an 8-bit key (LDS counting sort, unstable).
-    //! Thread t holds values t + j B (j < M) on entry; lane l of wave w
-    //! receives, as its value j, the value at sorted position 64 (M w + j) + l,
-    //! with the block slot it came from.  Three barriers: after clearing the
-    //! bucket counts, after counting, after scattering.  Between the last
-    //! two, every wave scans the 256 counts itself and writes the same bucket
+    //! Sorts the B x M values of a block (B = 256, 512 or 1024 threads, M
+    //! values per thread) by an 8-bit key into LDS (counting sort,
+    //! unstable): thread t holds values t + j B (j < M) on entry; on return
+    //! sortedValue[p] is the value at sorted position p and sortedSlot[p] the
+    //! block slot it came from.  Three barriers: after clearing the bucket
+    //! counts, after counting, after scattering.  Between the last two,
+    //! every wave scans the 256 counts itself and writes the same bucket
     //! starts to LDS (a wave reads only after its own writes, and every wave
     //! writes the same values), instead of four waves scanning and two more
     //! barriers to share their totals.
     template<class T, u32 B, u32 M>
-    __device__ void partitionByKeyMany(const u32 (&key)[M], const T (&value)[M], T (&outValue)[M],
-                                       u32 (&outSlot)[M])
+    __device__ void sortByKeyToLds(const u32 (&key)[M], const T (&value)[M], T* sortedValue, u32* sortedSlot)
     {
         static_assert(B % 256u == 0u && B <= 1024u, "a block of 256, 512 or 1024 threads");
         static_assert(kSizeBuckets == 4u * 64u, "four buckets per lane in the scan");
         __shared__ u32 count[kSizeBuckets];
         __shared__ u32 start[kSizeBuckets];
-        __shared__ T sortedValue[B * M];
-        __shared__ u32 sortedSlot[B * M];
         const u32 tid = threadIdx.x;
         const u32 me = hw::laneid();
         if(tid < kSizeBuckets)
@@ -130,7 +127,19 @@ namespace
             sortedSlot[pos] = tid + j * B;
         }
         __syncthreads();
-        const u32 first = (tid - me) * M + me; // 64 M w + l
+    }
+
+    //! Reorders the B x M values of a block by an 8-bit key
+    //! (sortByKeyToLds): lane l of wave w receives, as its value j, the value
+    //! at sorted position 64 (M w + j) + l, with the block slot it came from.
+    template<class T, u32 B, u32 M>
+    __device__ void partitionByKeyMany(const u32 (&key)[M], const T (&value)[M], T (&outValue)[M],
+                                       u32 (&outSlot)[M])
+    {
+        __shared__ T sortedValue[B * M];
+        __shared__ u32 sortedSlot[B * M];
+        sortByKeyToLds<T, B, M>(key, value, sortedValue, sortedSlot);
+        const u32 first = (threadIdx.x - hw::laneid()) * M + hw::laneid(); // 64 M w + l
 #pragma unroll
         for(u32 j = 0; j < M; ++j)
         {
@@ -205,6 +214,87 @@ namespace
         for(u32 j = 0; j < M; ++j)
             if(base + slot[j] < n)
                 offsets[base + slot[j]] = h.offsetOf(p[j]);
+    }
+
+    //! Two requests per lane, 1,024-thread blocks: each block sorts 4,096
+    //! requests by size bucket in LDS and its 16 waves serve them as 32
+    //! windows of 128 sorted requests (WaveAlloc::mallocStream), each wave
+    //! taking the next window from a counter in LDS until none is left.  A
+    //! wave that finishes early takes more windows, so the few windows of the
+    //! smallest size classes (several classes each, served one after the
+    //! other) no longer hold a block, and the next block's place on the CU,
+    //! after its other waves have ended; sorting 4,096 requests instead of
+    //! 2,048 also halves the windows that straddle two classes.  Windows go
+    //! out smallest sizes first: the multi-class windows start at once, and
+    //! the largest sizes, whose claims take the most pages, come last.
+    //! Geo and Tun: the heap configuration and the tuning the kernel is built
+    //! for (PinnedGeometry and DefaultTuning, or RuntimeGeometry and
+    //! RuntimeTuning for any).
+    template<class Geo, class Tun>
+    __global__ __launch_bounds__(1024) void waveMallocWindowsKernel(ScatterHeap view, WaveTuning tuning,
+                                                                   const u32* sizes, u64* offsets, u64 n)
+    {
+        constexpr u32 B = 1024u;   // threads
+        constexpr u32 kSort = 4u;  // requests a thread brings to the sort
+        constexpr u32 M = 2u;      // requests per lane in a window
+        constexpr u32 kWindow = 64u * M;
+        constexpr u32 kWindows = B * kSort / kWindow;
+        const ScatterHeap h = Geo::apply(view);
+        const WaveTuning t = Tun::apply(tuning);
+        __shared__ u32 sortedBytes[B * kSort];
+        __shared__ u32 sortedSlot[B * kSort];
+        __shared__ u32 windowsTaken;
+        const u64 base = static_cast<u64>(blockIdx.x) * (B * kSort);
+        if(threadIdx.x == 0u)
+            windowsTaken = 0u; // ordered before its first use by the sort's barriers
+        u32 key[kSort], bytes[kSort];
+#pragma unroll
+        for(u32 j = 0; j < kSort; ++j)
+        {
+            const u64 i = base + threadIdx.x + j * B;
+            bytes[j] = i < n ? sizes[i] : 0u;
+            key[j] = sizeBucket(bytes[j], h.pagesize);
+        }
+        sortByKeyToLds<u32, B, kSort>(key, bytes, sortedBytes, sortedSlot);
+        struct Windows
+        {
+            const ScatterHeap& h;
+            const u32* sortedBytes;
+            const u32* sortedSlot;
+            u32* taken;
+            u64* offsets;
+            u64 base;
+            u64 n;
+            u32 first; // the current window's first sorted position
+            __device__ bool next(u32 (&b)[M])
+            {
+                u32 k = 0u;
+                if(hw::laneid() == 0u)
+                    k = atomicAdd(taken, 1u);
+                k = wave::uniform(k);
+                if(k >= kWindows)
+                    return false;
+                first = (kWindows - 1u - k) * kWindow; // smallest sizes first
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                    b[j] = sortedBytes[first + 64u * j + hw::laneid()];
+                return true;
+            }
+            __device__ void done(void* const (&res)[M])
+            {
+                // slots are read from LDS here, not kept in registers
+#pragma unroll
+                for(u32 j = 0; j < M; ++j)
+                {
+                    const u32 slot = sortedSlot[first + 64u * j + hw::laneid()];
+                    if(base + slot < n)
+                        offsets[base + slot] = h.offsetOf(res[j]);
+                }
+            }
+        };
+        Windows src{h, sortedBytes, sortedSlot, &windowsTaken, offsets, base, n, 0u};
+        const WaveAlloc wa{h, t};
+        wa.mallocStream<M>(src);
     }
 
     //! With `partition` the block's frees are first reordered by page index
@@ -964,18 +1054,19 @@ int mmc_wave_malloc_batch(mmc_heap* heap, const uint32_t* d_sizes, uint64_t* d_o
                     heap->view,
                     [&](auto g) {
                         using G = decltype(g);
+                        const dim3 grid(mmc::gridFor(n, 4096u));
                         // the default tuning is pinned only beside a pinned geometry
                         if constexpr(!std::is_same_v<G, RuntimeGeometry>)
                         {
                             if(DefaultTuning::matches(t))
                             {
-                                hipLaunchKernelGGL((waveMallocManyKernel<1024, 2, G, DefaultTuning>), dim3(blocks2),
-                                                   dim3(1024), 0, s, heap->view, t, d_sizes, d_offsets, n);
+                                hipLaunchKernelGGL((waveMallocWindowsKernel<G, DefaultTuning>), grid, dim3(1024), 0,
+                                                   s, heap->view, t, d_sizes, d_offsets, n);
                                 return;
                             }
                         }
-                        hipLaunchKernelGGL((waveMallocManyKernel<1024, 2, G, RuntimeTuning>), dim3(blocks2),
-                                           dim3(1024), 0, s, heap->view, t, d_sizes, d_offsets, n);
+                        hipLaunchKernelGGL((waveMallocWindowsKernel<G, RuntimeTuning>), grid, dim3(1024), 0, s,
+                                           heap->view, t, d_sizes, d_offsets, n);
                     },
                     PinnedGeometries{});
             else if(bs == 512u)

@@ -190,9 +190,13 @@ def test_wave_overcommitted_batch(mmc, cuda, lane_requests):
     r = a.verify_pattern(offs, sizes, salt=4)
     assert r["mismatches"] == 0 and r["misaligned"] == 0 and r["outside"] == 0, r
     served = n - r["nulls"]
-    print(f"  served {served} of {n}", flush=True)
-    # the heap holds about 64 MiB of chunks: a small fraction of the ~540 MB asked for
-    assert 10000 < served < n // 4, r
+    served_bytes = int(sizes[offs >= 0].sum().item())
+    print(f"  served {served} of {n}, {served_bytes} B", flush=True)
+    # the heap holds about 64 MiB of chunks: a small fraction of the ~540 MB
+    # asked for (how many requests that is depends on which sizes were served
+    # first)
+    assert 10000 < served < n // 2, r
+    assert served_bytes <= 64 * MiB
     assert no_overlap(torch, offs, sizes)
     # the heap is full for the sizes asked: at most a few pages (under 1% of
     # the ~1,000) left free, which concurrent claims passed over
