@@ -276,11 +276,15 @@ PROFILE_NAME = os.path.relpath(COUNTERS_PROFILE, HERE)
 
 
 def counter_run(point):
-    """The committed counter passes at `point` (profiles/batched_mixed1M_counters.json), or None."""
+    """The committed counter passes at `point` (profiles/batched_mixed1M_counters.json), or None
+    when there are none for the kernels the point launches."""
     if not os.path.exists(COUNTERS_PROFILE):
         return None
     with open(COUNTERS_PROFILE) as f:
-        return json.load(f).get("runs", {}).get(point_key(point))
+        run = json.load(f).get("runs", {}).get(point_key(point))
+    if run is None or kernel_names(point)[0] not in run or "atomicsKernel" not in run:
+        return None
+    return run
 
 
 def point_key(point):
